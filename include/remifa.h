@@ -1,0 +1,232 @@
+/*
+ * remifa.h -- C ABI of libremifa_hip, the MI355X (gfx950) no-pivot LU library.
+ *
+ * Plain C: pointers, sizes and int status codes; no C++ or torch types cross
+ * this boundary. Matrices are column-major with a leading dimension, exactly
+ * as in the reference (flipflapflop/remifa). All device pointers are HIP
+ * device memory; every call enqueues work on the handle's stream and, like
+ * the reference drivers, the LU drivers synchronise that stream before
+ * returning (src/lu_nopiv_ll.hxx:567-569).
+ *
+ * Status codes: REMIFA_SUCCESS (0); -i when argument i (1-based, the handle
+ * counts as argument 1) is invalid; REMIFA_ERR_* (> 0) for runtime errors.
+ * remifa_last_error() returns a message for the last failure of the thread.
+ */
+#ifndef REMIFA_H
+#define REMIFA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define REMIFA_SUCCESS 0
+#define REMIFA_ERR_HIP 1000         /* a HIP runtime call failed */
+#define REMIFA_ERR_ALLOC 1001       /* device allocation failed */
+#define REMIFA_ERR_UNSUPPORTED 1002 /* valid but unsupported combination */
+#define REMIFA_ERR_WORKSPACE 1003   /* user workspace too small */
+
+typedef struct remifa_handle_st *remifa_handle_t;
+
+/* element types */
+typedef enum {
+  REMIFA_R_32F = 0,
+  REMIFA_R_16F = 1,
+  REMIFA_R_64F = 2,
+  REMIFA_R_16BF = 3,
+  REMIFA_R_8F_E4M3 = 4
+} remifa_dtype_t;
+
+/* update compute types. The first four are the reference's
+ * remifa::compute_type (src/utils.hxx:18-24):
+ *   FP32: fp32 operands and accumulation (cublasSgemm, src/gemm.cu:31)
+ *   FP16: fp16 operands and accumulation (cublasHgemm, src/gemm.cu:96)
+ *   TC32: fp16 operands, fp32 accumulation (GemmEx 32F, src/gemm.cu:240,276)
+ *   TC16: fp16 operands, fp16 accumulation (GemmEx 16F, src/gemm.cu:162)
+ * The MI355X additions feed the same fp32-accumulating matrix cores with
+ *   BF16_TC32: bf16 operands (rounded from the stored factors)
+ *   FP8_TC32:  OCP e4m3 operands, one power-of-two scale per 32-long K block
+ *              of every row of A and column of B (outer updates only). */
+typedef enum {
+  REMIFA_FP32 = 0,
+  REMIFA_FP16 = 1,
+  REMIFA_TC32 = 2,
+  REMIFA_TC16 = 3,
+  REMIFA_BF16_TC32 = 4,
+  REMIFA_FP8_TC32 = 5
+} remifa_compute_t;
+
+typedef enum { REMIFA_LL = 0, REMIFA_RL = 1 } remifa_algo_t;
+
+/* Precision tuple of the reference driver templates (src/lu_nopiv_ll.hxx:41-54):
+ * W storage, I/O inner/outer update compute types, OP/IP outer/inner panel
+ * (update accumulation) precisions, OF/IF outer/inner factor precisions, ib
+ * the inner (diagonal tile) block size. Precisions are remifa_dtype_t
+ * (REMIFA_R_32F or REMIFA_R_16F). */
+typedef struct {
+  int W;
+  int I, O;
+  int OP, IP;
+  int OF, IF;
+  int64_t ib;
+} remifa_lu_config_t;
+
+/* ------------------------------------------------------------ handle */
+/* Replaces the cublasHandle_t the reference drivers take
+ * (src/lu_nopiv_ll.hxx:56, :139 cublasGetStream). stream is a hipStream_t
+ * (NULL = the default stream). */
+int remifa_create(remifa_handle_t *handle, void *stream);
+int remifa_destroy(remifa_handle_t handle);
+int remifa_set_stream(remifa_handle_t handle, void *stream);
+int remifa_get_stream(remifa_handle_t handle, void **stream);
+/* Optional caller-owned workspace (device memory). When set, the library
+ * never allocates; calls needing more return REMIFA_ERR_WORKSPACE. */
+int remifa_set_workspace(remifa_handle_t handle, void *d_ws, size_t bytes);
+/* Bytes of workspace remifa_lu_nopiv needs for this problem. */
+int remifa_lu_workspace_size(remifa_handle_t handle, int algo, const remifa_lu_config_t *cfg,
+                             int64_t m, int64_t n, int64_t nb, size_t *bytes);
+/* Tuning knobs (0 = automatic): width of the fused panel kernels and the
+ * split-K factor of the left-looking updates. */
+int remifa_set_tuning(remifa_handle_t handle, int panel_width, int split_k);
+const char *remifa_last_error(void);
+
+/* Per-kernel timing with hipEvents on the handle's stream. While enabled,
+ * every launch of the library is bracketed by two events. class: 0 update
+ * GEMM, 1 panel, 2 convert, 3 triangular solve, 4 FP64 residual, 5 other. */
+int remifa_timing_enable(remifa_handle_t handle, int enable);
+/* Sums over the recorded launches of `cls`: total ms, launches and work
+ * (flops for GEMM/TRSM/residual, bytes for panel/convert). Synchronises. */
+int remifa_timing_query(remifa_handle_t handle, int cls, double *ms, int64_t *launches,
+                        double *work);
+int remifa_timing_reset(remifa_handle_t handle);
+
+/* ------------------------------------------------------------ drivers */
+/* Generic no-pivot LU of the m x n (m == n) column-major matrix d_a.
+ * algo REMIFA_LL: lu_nopiv_ll<W,I,O,ib,OP,IP,OF,IF> (src/lu_nopiv_ll.hxx:55);
+ * algo REMIFA_RL: lu_nopiv_rl<W,I,O,ib> (src/lu_nopiv.hxx:53; OP..IF = W).
+ * d_a_cpy/ld_a_cpy: fp16 copy of the factors the reference requires when
+ * W = fp32 and an update is TC32 (src/lu_nopiv_ll.hxx:132-135); pass NULL to
+ * let the library use workspace instead. d_info (device int) receives 0, or
+ * k > 0 when u_kk of column k (1-based) is zero or not finite. */
+int remifa_lu_nopiv(remifa_handle_t handle, int algo, const remifa_lu_config_t *cfg, int64_t m,
+                    int64_t n, int64_t nb, void *d_a, int64_t ldda, int *d_info, void *d_a_cpy,
+                    int64_t ld_a_cpy);
+
+/* The 12 cuBLAS-backed helpers of src/lu_nopiv_ll.cxx (ib = 8). Same names
+ * (with the remifa_ prefix), arguments and precision tuples. */
+/* src/lu_nopiv_ll.cxx:14   W=f32 I=O=FP32 */
+int remifa_lu_nopiv_ll_f32(remifa_handle_t h, int m, int n, int nb, float *d_a, int ldda,
+                           int *d_info);
+/* src/lu_nopiv_ll.cxx:74   W=f32 I=FP32 O=TC32 */
+int remifa_lu_nopiv_ll_f32_fp32xtc32(remifa_handle_t h, int m, int n, int nb, float *d_a,
+                                     int ldda, int *d_info, void *d_a_f16, int ld_a_f16);
+/* src/lu_nopiv_ll.cxx:131  W=f32 I=O=TC32 */
+int remifa_lu_nopiv_ll_f32_tc32xtc32(remifa_handle_t h, int m, int n, int nb, float *d_a,
+                                     int ldda, int *d_info, void *d_a_f16, int ld_a_f16);
+/* src/lu_nopiv_ll.cxx:191  W=f16 I=O=FP16 */
+int remifa_lu_nopiv_ll_f16(remifa_handle_t h, int m, int n, int nb, void *d_a, int ldda,
+                           int *d_info);
+/* src/lu_nopiv_ll.cxx:246  W=f16 I=O=TC16 */
+int remifa_lu_nopiv_ll_f16_tc16xtc16(remifa_handle_t h, int m, int n, int nb, void *d_a,
+                                     int ldda, int *d_info);
+/* src/lu_nopiv_ll.cxx:282  W=f16 I=O=TC32 */
+int remifa_lu_nopiv_ll_f16_tc32xtc32(remifa_handle_t h, int m, int n, int nb, void *d_a,
+                                     int ldda, int *d_info);
+/* src/lu_nopiv_ll.cxx:338  W=f16 I=FP16 O=TC32 */
+int remifa_lu_nopiv_ll_f16_f16xtc32(remifa_handle_t h, int m, int n, int nb, void *d_a,
+                                    int ldda, int *d_info);
+/* src/lu_nopiv_ll.cxx:393  W=f16 I=FP16 O=TC32 OP=f32 */
+int remifa_lu_nopiv_ll_f16_f16xtc32_fp32(remifa_handle_t h, int m, int n, int nb, void *d_a,
+                                         int ldda, int *d_info);
+/* src/lu_nopiv_ll.cxx:448  W=f16 I=FP32 O=TC32 OP=IP=OF=IF=f32 */
+int remifa_lu_nopiv_ll_f16_f32xt32(remifa_handle_t h, int m, int n, int nb, void *d_a, int ldda,
+                                   int *d_info);
+/* src/lu_nopiv_ll.cxx:507  W=f16 I=FP16 O=TC16 */
+int remifa_lu_nopiv_ll_f16_f16xtc16(remifa_handle_t h, int m, int n, int nb, void *d_a,
+                                    int ldda, int *d_info);
+/* src/lu_nopiv_ll.cxx:563  W=f16 I=O=TC32 OP=OF=f16 IP=IF=f32 */
+int remifa_lu_nopiv_ll_f16_tc32xtc32_fp16xfp32_fp16xfp32(remifa_handle_t h, int m, int n, int nb,
+                                                         void *d_a, int ldda, int *d_info);
+/* src/lu_nopiv_ll.cxx:618  W=f16 I=O=TC32 OP=IP=OF=IF=f32 */
+int remifa_lu_nopiv_ll_f16_tc32xtc32_fp32xfp32_fp32xfp32(remifa_handle_t h, int m, int n, int nb,
+                                                         void *d_a, int ldda, int *d_info);
+
+/* ------------------------------------------------------------ kernels */
+/* src/lu_nopiv.cuh:12-20 / src/lu_nopiv.cu:364-400 lu_nopiv_panel<T,ib>:
+ * factor the n x n diagonal tile d (n <= ib <= 32), write it to l, scale and
+ * update the rows n..m-1 of the block column l (L21 = A21 U11^-1) and solve
+ * the columns n..m-1 of the block row u (U12 = L11^-1 A12). t is
+ * REMIFA_R_32F, REMIFA_R_16F or REMIFA_R_64F. stat[0] = n on return, as in
+ * the reference (:173). */
+int remifa_lu_nopiv_panel(remifa_handle_t h, int t, int ib, int m, int n, const void *d, int ldd,
+                          void *l, int ldl, void *u, int ldu, int *stat);
+
+/* src/convert.cuh:17-18 / src/convert.cu:324-360 convert<TA,TAO>: cast an
+ * m x n block, round to nearest even when narrowing. Any pair of
+ * remifa_dtype_t; src == dst copies (the reference's same-type convert is a
+ * no-op). */
+int remifa_convert(remifa_handle_t h, int src_t, int dst_t, int64_t m, int64_t n, const void *a,
+                   int64_t lda, void *aout, int64_t ldaout);
+
+/* src/gemm.cuh:24-40 gemmex<Impl,Op,AccType,InType,OutType>: C = alpha A B +
+ * beta C (no transposes). ctype is a remifa_compute_t, in_t the storage type
+ * of A and B (f32 for FP32; f16 for FP16/TC16/TC32/BF16_TC32), out_t the
+ * storage type of C (f32 or f16; f16 for FP16/TC16). */
+int remifa_gemmex(remifa_handle_t h, int ctype, int in_t, int out_t, int m, int n, int k,
+                  float alpha, const void *a, int lda, const void *b, int ldb, float beta, void *c,
+                  int ldc);
+
+/* ------------------------------------------------------------ solves */
+/* Identity-pivot getrs (the reference's check solve host_getrs with
+ * ipiv = 1..n, tests/testing_lu_gpu.hxx:1309 -> src/wrappers.cxx:477), on the
+ * GPU: d_b (n x nrhs, fp32) <- (L U)^-1 d_b, or (L U)^-T d_b when trans != 0.
+ * The factors are lu_t (REMIFA_R_32F or REMIFA_R_16F); arithmetic is fp32. */
+int remifa_getrs_nopiv(remifa_handle_t h, int lu_t, int trans, int64_t n, int64_t nrhs,
+                       const void *d_lu, int64_t ldlu, float *d_b, int64_t ldb);
+
+/* FP64 residual R = B - A X (A fp32 n x n, X/B/R fp64 n x nrhs). No
+ * reference counterpart (the reference checks on the host in FP64,
+ * tests/errors.hxx:186-249). */
+int remifa_residual_f64(remifa_handle_t h, int64_t n, int64_t nrhs, const float *d_a,
+                        int64_t lda, const double *d_x, int64_t ldx, const double *d_b,
+                        int64_t ldb, double *d_r, int64_t ldr);
+
+/* Iterative refinement: X0 = (LU)^-1 B in fp32, then X += (LU)^-1 (B - A X)
+ * with the residual in FP64 until the normwise backward error of every
+ * column (tests/errors.hxx:186-249) is <= tol or maxit corrections. Returns
+ * the corrections applied in *iters (-1 if not converged) and the final
+ * backward error in *bwd_err. No reference counterpart. */
+int remifa_ir_solve(remifa_handle_t h, int lu_t, int64_t n, int64_t nrhs, const float *d_a,
+                    int64_t lda, const void *d_lu, int64_t ldlu, const double *d_b, int64_t ldb,
+                    double *d_x, int64_t ldx, double tol, int maxit, int *iters, double *bwd_err,
+                    double *history);
+
+/* ------------------------------------------------------------ generators */
+/* The benchmark generator on the GPU: a_ij uniform(-1,1) drawn row by row
+ * from libstdc++'s minstd_rand0 seeded with seed (tests/genmat.hxx:68-102
+ * draws the same way), then a_ii = fp32(alpha * sum_{j!=i} |a_ij|) (FP64
+ * sum). alpha <= 0 gives gen_diagdom itself: a_ii = n, default seed. */
+int remifa_gen_diagdom(remifa_handle_t h, int64_t n, uint64_t seed, double alpha, float *d_a,
+                       int64_t lda);
+/* Seeded uniform(-1,1) m x n block, row-major draw order. */
+int remifa_gen_uniform(remifa_handle_t h, int64_t m, int64_t n, uint64_t seed, float *d_b,
+                       int64_t ldb);
+
+/* ------------------------------------------------------------ batched */
+/* Factor `count` independent matrices of orders n[i] (<= 512) stored at
+ * d_ptrs[i] (fp32, leading dimension ld[i]) in place, in one persistent
+ * launch. d_n, d_ld, d_ptrs are device arrays. No reference counterpart. */
+int remifa_lu_nopiv_batched(remifa_handle_t h, int ctype, int64_t count, const int *d_n,
+                            const int *d_ld, float *const *d_ptrs, int *d_info, int max_n);
+/* Solve A_i x_i = b_i for every matrix of the batch with its factors
+ * (fp32 right-hand sides, one per matrix, overwritten). */
+int remifa_getrs_nopiv_batched(remifa_handle_t h, int64_t count, const int *d_n, const int *d_ld,
+                               float *const *d_ptrs, float *const *d_b, int max_n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,262 @@
+"""Torch-tensor front end of the C ABI.
+
+Names and argument order follow the reference's C++ entry points
+(src/lu_nopiv_ll.hxx:588-793): ``lu_nopiv_ll_f32(handle, m, n, nb, d_a, ldda,
+d_info)`` etc., where ``d_a`` is a column-major device tensor and ``d_info``
+a one-element int32 device tensor. Errors raise RemifaError (a RuntimeError;
+negative statuses name the invalid argument), as the reference raises
+std::invalid_argument / std::runtime_error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from ._lib import (R_32F, R_16F, R_64F, R_16BF, R_8F_E4M3, FP32, FP16, TC32, TC16, BF16_TC32,
+                   FP8_TC32, LL, RL, LuConfig, check, lib)
+
+_DT = {torch.float32: R_32F, torch.float16: R_16F, torch.float64: R_64F,
+       torch.bfloat16: R_16BF, torch.float8_e4m3fn: R_8F_E4M3}
+
+
+def dtype_code(t) -> int:
+    dt = t.dtype if isinstance(t, torch.Tensor) else t
+    if dt not in _DT:
+        raise TypeError(f"unsupported dtype {dt}")
+    return _DT[dt]
+
+
+def colmajor_empty(m: int, n: int, dtype=torch.float32, device="cuda", ld: int | None = None):
+    """m x n column-major tensor view (stride (1, ld))."""
+    ld = max(m, 1) if ld is None else ld
+    return torch.empty((n, ld), dtype=dtype, device=device).t()[:m]
+
+
+def to_colmajor(a: torch.Tensor, dtype=None, device=None):
+    dtype = a.dtype if dtype is None else dtype
+    device = a.device if device is None else device
+    out = colmajor_empty(a.shape[0], a.shape[1], dtype=dtype, device=device)
+    out.copy_(a)
+    return out
+
+
+def _ld(a: torch.Tensor) -> int:
+    if a.dim() == 1:
+        return max(a.shape[0], 1)
+    if a.stride(0) != 1 and a.shape[0] > 1:
+        raise ValueError("matrix must be column-major (stride(0) == 1)")
+    if a.shape[1] <= 1:
+        return max(a.stride(1), a.shape[0], 1)
+    return a.stride(1)
+
+
+def _p(t) -> C.c_void_p:
+    if t is None:
+        return None
+    return C.c_void_p(t.data_ptr())
+
+
+class Handle:
+    """remifa_handle_t bound to a torch CUDA (HIP) stream."""
+
+    def __init__(self, stream: torch.cuda.Stream | None = None):
+        self._h = C.c_void_p()
+        s = torch.cuda.current_stream() if stream is None else stream
+        self.stream = s
+        check(lib().remifa_create(C.byref(self._h), C.c_void_p(s.cuda_stream)), "remifa_create")
+
+    @property
+    def value(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().remifa_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream: torch.cuda.Stream):
+        self.stream = stream
+        check(lib().remifa_set_stream(self._h, C.c_void_p(stream.cuda_stream)), "remifa_set_stream")
+
+    def set_workspace(self, ws: torch.Tensor | None):
+        nbytes = 0 if ws is None else ws.numel() * ws.element_size()
+        check(lib().remifa_set_workspace(self._h, _p(ws), nbytes), "remifa_set_workspace")
+        self._ws = ws
+
+    def set_tuning(self, panel_width: int = 0, split_k: int = 0):
+        check(lib().remifa_set_tuning(self._h, panel_width, split_k), "remifa_set_tuning")
+
+    # per-kernel timing -------------------------------------------------
+    def timing(self, enable: bool = True):
+        check(lib().remifa_timing_enable(self._h, int(enable)), "remifa_timing_enable")
+
+    def timing_reset(self):
+        check(lib().remifa_timing_reset(self._h), "remifa_timing_reset")
+
+    def timing_query(self, cls: int):
+        ms, n, w = C.c_double(), C.c_int64(), C.c_double()
+        check(lib().remifa_timing_query(self._h, cls, C.byref(ms), C.byref(n), C.byref(w)),
+              "remifa_timing_query")
+        return ms.value, n.value, w.value
+
+
+def _h(handle) -> C.c_void_p:
+    return handle.value if isinstance(handle, Handle) else handle
+
+
+# ------------------------------------------------------------------ drivers
+def lu_workspace_size(handle, algo: int, cfg: LuConfig, m: int, n: int, nb: int) -> int:
+    out = C.c_size_t()
+    check(lib().remifa_lu_workspace_size(_h(handle), algo, C.byref(cfg), m, n, nb, C.byref(out)),
+          "remifa_lu_workspace_size")
+    return out.value
+
+
+def lu_nopiv(handle, a: torch.Tensor, nb: int, algo: int = LL, cfg: LuConfig | None = None,
+             info: torch.Tensor | None = None, a_cpy: torch.Tensor | None = None, **kw) -> int:
+    """In-place no-pivot LU of the square column-major tensor `a` (the
+    generic lu_nopiv_ll / lu_nopiv_rl driver). Returns the info value (0, or
+    the 1-based column of the first zero / non-finite pivot)."""
+    if cfg is None:
+        cfg = make_config(W=dtype_code(a), **kw)
+    m, n = a.shape
+    own_info = info is None
+    if own_info:
+        info = torch.zeros(1, dtype=torch.int32, device=a.device)
+    ldc = _ld(a_cpy) if a_cpy is not None else 0
+    check(lib().remifa_lu_nopiv(_h(handle), algo, C.byref(cfg), m, n, nb, _p(a), _ld(a), _p(info),
+                                _p(a_cpy), ldc), "remifa_lu_nopiv")
+    return int(info.item()) if own_info else 0
+
+
+def make_config(W=R_32F, I=FP32, O=FP32, OP=None, IP=None, OF=None, IF=None, ib=8) -> LuConfig:
+    OP = W if OP is None else OP
+    IP = OP if IP is None else IP
+    OF = OP if OF is None else OF
+    IF = OF if IF is None else IF
+    return LuConfig(W, I, O, OP, IP, OF, IF, ib)
+
+
+def _helper(name, with_cpy):
+    def f(handle, m, n, nb, d_a, ldda, d_info, d_a_f16=None, ld_a_f16=None):
+        fn = getattr(lib(), "remifa_" + name)
+        if with_cpy:
+            ldc = ld_a_f16 if ld_a_f16 is not None else (_ld(d_a_f16) if d_a_f16 is not None else 0)
+            st = fn(_h(handle), m, n, nb, _p(d_a), ldda, _p(d_info), _p(d_a_f16), ldc)
+        else:
+            st = fn(_h(handle), m, n, nb, _p(d_a), ldda, _p(d_info))
+        check(st, name)
+    f.__name__ = name
+    f.__doc__ = f"remifa::{name} (src/lu_nopiv_ll.cxx) on the GPU."
+    return f
+
+
+# (name, takes the fp16 copy, W, I, O, OP, IP, OF, IF): src/lu_nopiv_ll.cxx
+REFERENCE_HELPERS = [
+    ("lu_nopiv_ll_f32", False, R_32F, FP32, FP32, R_32F, R_32F, R_32F, R_32F),
+    ("lu_nopiv_ll_f32_fp32xtc32", True, R_32F, FP32, TC32, R_32F, R_32F, R_32F, R_32F),
+    ("lu_nopiv_ll_f32_tc32xtc32", True, R_32F, TC32, TC32, R_32F, R_32F, R_32F, R_32F),
+    ("lu_nopiv_ll_f16", False, R_16F, FP16, FP16, R_16F, R_16F, R_16F, R_16F),
+    ("lu_nopiv_ll_f16_tc16xtc16", False, R_16F, TC16, TC16, R_16F, R_16F, R_16F, R_16F),
+    ("lu_nopiv_ll_f16_tc32xtc32", False, R_16F, TC32, TC32, R_16F, R_16F, R_16F, R_16F),
+    ("lu_nopiv_ll_f16_f16xtc32", False, R_16F, FP16, TC32, R_16F, R_16F, R_16F, R_16F),
+    ("lu_nopiv_ll_f16_f16xtc32_fp32", False, R_16F, FP16, TC32, R_16F, R_16F, R_16F, R_16F),
+    ("lu_nopiv_ll_f16_f32xt32", False, R_16F, FP32, TC32, R_32F, R_32F, R_32F, R_32F),
+    ("lu_nopiv_ll_f16_f16xtc16", False, R_16F, FP16, TC16, R_16F, R_16F, R_16F, R_16F),
+    ("lu_nopiv_ll_f16_tc32xtc32_fp16xfp32_fp16xfp32", False, R_16F, TC32, TC32, R_16F, R_32F,
+     R_16F, R_32F),
+    ("lu_nopiv_ll_f16_tc32xtc32_fp32xfp32_fp32xfp32", False, R_16F, TC32, TC32, R_32F, R_32F,
+     R_32F, R_32F),
+]
+
+for _row in REFERENCE_HELPERS:
+    globals()[_row[0]] = _helper(_row[0], _row[1])
+
+
+# ------------------------------------------------------------------ kernels
+def lu_nopiv_panel(handle, ib: int, m: int, n: int, d: torch.Tensor, l: torch.Tensor,
+                   u: torch.Tensor, stat: torch.Tensor | None = None):
+    """lu_nopiv_panel<T, ib> (src/lu_nopiv.cu:364-400) on column-major views."""
+    check(lib().remifa_lu_nopiv_panel(_h(handle), dtype_code(d), ib, m, n, _p(d), _ld(d), _p(l),
+                                      _ld(l), _p(u), _ld(u), _p(stat)), "remifa_lu_nopiv_panel")
+
+
+def convert(handle, a: torch.Tensor, out: torch.Tensor):
+    """convert<TA,TAO> (src/convert.cu:324-360): out <- a, rounded."""
+    m, n = a.shape
+    check(lib().remifa_convert(_h(handle), dtype_code(a), dtype_code(out), m, n, _p(a), _ld(a),
+                               _p(out), _ld(out)), "remifa_convert")
+
+
+def gemmex(handle, ctype: int, a: torch.Tensor, b: torch.Tensor, c: torch.Tensor,
+           alpha: float = -1.0, beta: float = 1.0):
+    """gemmex (src/gemm.cuh:24-40): c <- alpha a b + beta c."""
+    m, k = a.shape
+    n = b.shape[1]
+    check(lib().remifa_gemmex(_h(handle), ctype, dtype_code(a), dtype_code(c), m, n, k, alpha,
+                              _p(a), _ld(a), _p(b), _ld(b), beta, _p(c), _ld(c)), "remifa_gemmex")
+
+
+# ------------------------------------------------------------------ solves
+def getrs_nopiv(handle, lu: torch.Tensor, b: torch.Tensor, trans: bool = False):
+    """b (fp32, n x nrhs column-major) <- (LU)^-1 b or (LU)^-T b."""
+    n = lu.shape[0]
+    nrhs = b.shape[1] if b.dim() == 2 else 1
+    check(lib().remifa_getrs_nopiv(_h(handle), dtype_code(lu), int(trans), n, nrhs, _p(lu),
+                                   _ld(lu), _p(b), _ld(b)), "remifa_getrs_nopiv")
+
+
+def residual_f64(handle, a: torch.Tensor, x: torch.Tensor, b: torch.Tensor, r: torch.Tensor):
+    n = a.shape[0]
+    nrhs = b.shape[1] if b.dim() == 2 else 1
+    check(lib().remifa_residual_f64(_h(handle), n, nrhs, _p(a), _ld(a), _p(x), _ld(x), _p(b),
+                                    _ld(b), _p(r), _ld(r)), "remifa_residual_f64")
+
+
+def ir_solve(handle, a: torch.Tensor, lu: torch.Tensor, b: torch.Tensor, x: torch.Tensor,
+             tol: float = 1e-14, maxit: int = 30):
+    """Iterative refinement (FP64 residual). Returns (iters, backward_error,
+    history); iters is -1 if tol was not reached."""
+    n = a.shape[0]
+    nrhs = b.shape[1] if b.dim() == 2 else 1
+    it, be = C.c_int(), C.c_double()
+    hist = (C.c_double * (maxit + 1))()
+    check(lib().remifa_ir_solve(_h(handle), dtype_code(lu), n, nrhs, _p(a), _ld(a), _p(lu),
+                                _ld(lu), _p(b), _ld(b), _p(x), _ld(x), tol, maxit, C.byref(it),
+                                C.byref(be), C.cast(hist, C.c_void_p)), "remifa_ir_solve")
+    k = it.value if it.value >= 0 else maxit
+    return it.value, be.value, list(hist)[: k + 1]
+
+
+# ------------------------------------------------------------------ generators
+def gen_diagdom(handle, a: torch.Tensor, seed: int = 1, alpha: float = 0.0):
+    """Fill the n x n fp32 column-major `a`: uniform(-1,1) off-diagonal from the
+    seeded minstd_rand0 stream (row-major draws), a_ii = alpha * sum_j!=i |a_ij|
+    (alpha <= 0: gen_diagdom of tests/genmat.hxx, a_ii = n, default seed)."""
+    n = a.shape[0]
+    check(lib().remifa_gen_diagdom(_h(handle), n, seed, alpha, _p(a), _ld(a)), "remifa_gen_diagdom")
+
+
+def gen_uniform(handle, b: torch.Tensor, seed: int):
+    m, n = b.shape
+    check(lib().remifa_gen_uniform(_h(handle), m, n, seed, _p(b), _ld(b)), "remifa_gen_uniform")
+
+
+# ------------------------------------------------------------------ batched
+def lu_nopiv_batched(handle, ctype: int, n: torch.Tensor, ld: torch.Tensor, ptrs: torch.Tensor,
+                     info: torch.Tensor, max_n: int):
+    check(lib().remifa_lu_nopiv_batched(_h(handle), ctype, n.numel(), _p(n), _p(ld), _p(ptrs),
+                                        _p(info), max_n), "remifa_lu_nopiv_batched")
+
+
+def getrs_nopiv_batched(handle, n: torch.Tensor, ld: torch.Tensor, ptrs: torch.Tensor,
+                        bptrs: torch.Tensor, max_n: int):
+    check(lib().remifa_getrs_nopiv_batched(_h(handle), n.numel(), _p(n), _p(ld), _p(ptrs),
+                                           _p(bptrs), max_n), "remifa_getrs_nopiv_batched")

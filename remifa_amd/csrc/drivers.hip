@@ -1,0 +1,337 @@
+// Left-looking and right-looking no-pivot LU drivers.
+//
+// LL (src/lu_nopiv_ll.hxx:55-582 with src/lu_nopiv_inner_ll.hxx:37-372) and
+// RL (src/lu_nopiv.hxx:53-208 with src/lu_nopiv_inner_rl.hxx:34-198) keep
+// the reference's loop structure and rounding points: for every outer block
+// an update GEMM (LL: K = ofs, RL: the Schur complement with K = nb), an
+// inner loop over ib-wide panels (diagonal-tile snapshot, panel kernel,
+// ib-wide updates) and the fp16 copies feeding the TC32 updates.
+//
+// Differences from the reference, all deliberate:
+//  * no allocation inside the loop: one workspace arena per call (snapshot,
+//    fp32 panel buffers, fp16 copy when none is passed, split-K partials);
+//  * the precision tuple is a runtime argument (no template explosion);
+//  * d_info is 0 or the 1-based column of the first zero / non-finite pivot
+//    (the reference writes the last panel width, src/lu_nopiv.cu:173);
+//  * the inner U update reads and writes from column iofs + cblkn (the
+//    reference reads B from iofs + ib, which only agrees when nb % ib == 0).
+#include <algorithm>
+
+#include "common.hpp"
+
+namespace remifa {
+namespace {
+
+__global__ void info_kernel(int64_t n, const void* a, int64_t lda, int t, int* info) {
+  // one workgroup: first column k with u_kk == 0 or not finite (1-based)
+  __shared__ int first;
+  if (threadIdx.x == 0) first = 0x7fffffff;
+  __syncthreads();
+  for (int64_t k = threadIdx.x; k < n; k += blockDim.x) {
+    float v = (t == REMIFA_R_16F) ? (float)((const half_t*)a)[k + k * lda]
+                                  : ((const float*)a)[k + k * lda];
+    if (v == 0.0f || !__builtin_isfinite(v)) atomicMin(&first, (int)(k + 1));
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *info = first == 0x7fffffff ? 0 : first;
+}
+
+bool is_tc(int c) { return c == REMIFA_TC32 || c == REMIFA_BF16_TC32 || c == REMIFA_FP8_TC32; }
+bool is_h16acc(int c) { return c == REMIFA_FP16 || c == REMIFA_TC16; }
+
+// storage type of the operands a compute type reads
+int operand_type(int c) { return c == REMIFA_FP32 ? REMIFA_R_32F : REMIFA_R_16F; }
+
+struct Arena {
+  char* base;
+  size_t off = 0;
+  void* take(size_t bytes) {
+    void* p = base + off;
+    off += (bytes + 255) & ~(size_t)255;
+    return p;
+  }
+};
+
+constexpr size_t kGemmReserve = (size_t)64 << 20;  // split-K partials
+
+struct Ctx {
+  remifa_handle_t h;
+  remifa_lu_config_t c;
+  int64_t ib;
+  void* dsnap;  // ib x ib diagonal-tile snapshot
+};
+
+int gemm(Ctx& x, int ctype, int out_t, int64_t m, int64_t n, int64_t k, const void* a,
+         int64_t lda, const void* b, int64_t ldb, void* c, int64_t ldc) {
+  if (m <= 0 || n <= 0 || k <= 0) return 0;
+  int in_t = operand_type(ctype);
+  if (ctype == REMIFA_FP8_TC32) ctype = REMIFA_TC32;  // fp8 handled by the fused path only
+  return gemm_launch(x.h, ctype, in_t, out_t, m, n, k, -1.0f, a, lda, b, ldb, 1.0f, c, ldc);
+}
+
+// element pointer arithmetic on untyped buffers
+inline void* at(void* p, int t, int64_t i, int64_t j, int64_t ld) {
+  return (char*)p + (i + j * ld) * (int64_t)dtype_size(t);
+}
+
+// inner left-looking factorization of one outer block: m x n block column
+// `l` and n x m block row `u` (columns n.. beyond the diagonal block), both
+// in precision T (l == u when factoring in place).
+// cpy/ldcp: fp16 mirror of this block's factors. When T = fp32 and the inner
+// update is a TC type its operands are read from the mirror (the reference's
+// d_a_cpy, src/lu_nopiv_inner_ll.hxx:225-265); `mirror` asks for the new
+// factors to be written to it after every panel.
+int inner_ll(Ctx& x, int T, int64_t m, int64_t n, void* l, int64_t ldl, void* u, int64_t ldu,
+             void* cpy, int64_t ldcp, bool mirror) {
+  const int I = x.c.I;
+  const int64_t ib = x.ib;
+  const int64_t inc = (n - 1) / ib + 1;
+  const bool tc_from_copy = (T == REMIFA_R_32F) && I != REMIFA_FP32;
+  if (tc_from_copy) mirror = true;
+  for (int64_t k = 0; k < inc; ++k) {
+    const int64_t iofs = k * ib, cblkm = m - iofs;
+    const int64_t cblkn = std::min(n - iofs, ib);
+    if (iofs > 0) {
+      const void* la = tc_from_copy ? at(cpy, REMIFA_R_16F, iofs, 0, ldcp) : at(l, T, iofs, 0, ldl);
+      const int64_t lla = tc_from_copy ? ldcp : ldl;
+      const void* ub = tc_from_copy ? at(cpy, REMIFA_R_16F, 0, iofs, ldcp) : at(u, T, 0, iofs, ldu);
+      const int64_t lub = tc_from_copy ? ldcp : ldu;
+      // L update (:223-246)
+      RF_CHECK(gemm(x, I, T, cblkm, cblkn, iofs, la, lla, ub, lub, at(l, T, iofs, iofs, ldl), ldl));
+      // U update (:249-301)
+      if (cblkm > cblkn) {
+        const void* ub2 = tc_from_copy ? at(cpy, REMIFA_R_16F, 0, iofs + cblkn, ldcp)
+                                       : at(u, T, 0, iofs + cblkn, ldu);
+        RF_CHECK(gemm(x, I, T, cblkn, cblkm - cblkn, iofs, la, lla, ub2, lub,
+                      at(u, T, iofs, iofs + cblkn, ldu), ldu));
+      }
+    }
+    // diagonal snapshot + panel (:313-360)
+    RF_CHECK(copy2d_launch(x.h, dtype_size(T), cblkn, cblkn, at(l, T, iofs, iofs, ldl), ldl, x.dsnap,
+                           ib));
+    RF_CHECK(panel_launch(x.h, T, (int)ib, cblkm, cblkn, x.dsnap, ib, at(l, T, iofs, iofs, ldl), ldl,
+                          at(u, T, iofs, iofs, ldu), ldu, nullptr));
+    // fp16 copies of the new factors (:362-368)
+    if (mirror) {
+      if (cblkm > cblkn)
+        RF_CHECK(convert_launch(x.h, T, REMIFA_R_16F, cblkn, cblkm - cblkn,
+                                at(u, T, iofs, iofs + cblkn, ldu), ldu,
+                                at(cpy, REMIFA_R_16F, iofs, iofs + cblkn, ldcp), ldcp));
+      RF_CHECK(convert_launch(x.h, T, REMIFA_R_16F, cblkm, cblkn, at(l, T, iofs, iofs, ldl), ldl,
+                              at(cpy, REMIFA_R_16F, iofs, iofs, ldcp), ldcp));
+    }
+  }
+  return 0;
+}
+
+int validate(const remifa_lu_config_t& c, int algo) {
+  auto prec_ok = [](int p) { return p == REMIFA_R_32F || p == REMIFA_R_16F; };
+  auto ct_ok = [](int t) { return t >= REMIFA_FP32 && t <= REMIFA_FP8_TC32; };
+  if (!prec_ok(c.W) || !ct_ok(c.I) || !ct_ok(c.O)) return 1;
+  if (!prec_ok(c.OP) || !prec_ok(c.IP) || !prec_ok(c.OF) || !prec_ok(c.IF)) return 1;
+  if (c.ib < 1 || c.ib > 32) return 1;
+  if (c.W == REMIFA_R_32F) {
+    // src/lu_nopiv_inner_ll.hxx:74-100: fp32 storage needs fp32 panels/factors
+    if (c.OP != REMIFA_R_32F || c.IP != REMIFA_R_32F || c.OF != REMIFA_R_32F ||
+        c.IF != REMIFA_R_32F)
+      return 1;
+    if (is_h16acc(c.I) || is_h16acc(c.O)) return 1;  // FP16/TC16 need fp16 storage
+  } else {
+    if (c.I == REMIFA_FP32 && c.IP != REMIFA_R_32F) return 1;  // :102-104
+    if (c.O == REMIFA_FP32) return 1;  // no fp32 GEMM on fp16 operands
+    if (algo == REMIFA_LL) {
+      // supported layouts: everything fp16 (in place) or fp32 inner buffers
+      bool inplace = c.IP == REMIFA_R_16F && c.IF == REMIFA_R_16F && c.OF == REMIFA_R_16F;
+      bool buffered = c.IP == REMIFA_R_32F && c.IF == REMIFA_R_32F && c.OF == c.OP;
+      if (!inplace && !buffered) return 1;
+      if (is_h16acc(c.O) && c.OP != REMIFA_R_16F) return 1;
+    }
+  }
+  // bf16 / fp8 updates read operands rounded from the fp16 storage
+  auto lowp = [](int t) { return t == REMIFA_BF16_TC32 || t == REMIFA_FP8_TC32; };
+  if (c.W == REMIFA_R_32F && (lowp(c.I) || lowp(c.O))) return 1;
+  if (c.I == REMIFA_FP8_TC32 || c.O == REMIFA_FP8_TC32) return 2;  // not built yet
+  return 0;
+}
+
+}  // namespace
+
+// Workspace needed by lu_nopiv (bytes).
+size_t lu_workspace_bytes(int algo, const remifa_lu_config_t& c, int64_t m, int64_t n, int64_t nb,
+                          bool have_cpy) {
+  size_t b = 256 + 32 * 32 * 8;  // snapshot
+  bool need_cpy = c.W == REMIFA_R_32F && (c.I != REMIFA_FP32 || c.O != REMIFA_FP32);
+  if (need_cpy && !have_cpy) b += ((size_t)m * n * 2 + 255) & ~(size_t)255;
+  if (algo == REMIFA_LL && c.W == REMIFA_R_16F && c.IP == REMIFA_R_32F) {
+    b += ((size_t)m * nb * 4 + 255) & ~(size_t)255;  // block column buffer
+    b += ((size_t)nb * m * 4 + 255) & ~(size_t)255;  // block row buffer
+  }
+  return b + kGemmReserve;
+}
+
+int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, int64_t m, int64_t n,
+                  int64_t nb, void* d_a, int64_t ldda, int* d_info, void* d_cpy, int64_t ldcpy) {
+  if (!h) return -1;
+  if (algo != REMIFA_LL && algo != REMIFA_RL) return -2;
+  if (!cfg || validate(*cfg, algo)) return -3;
+  if (m < 0) return -4;
+  if (n != m) return -5;  // the reference drivers assume square matrices
+  if (nb <= 0) return -6;
+  if (!d_a && m > 0) return -7;
+  if (ldda < std::max<int64_t>(m, 1)) return -8;
+  if (d_cpy && ldcpy < std::max<int64_t>(m, 1)) return -11;
+  const remifa_lu_config_t c = *cfg;
+  const int W = c.W;
+  if (m == 0) {
+    if (d_info) RF_HIP(hipMemsetAsync(d_info, 0, sizeof(int), h->stream));
+    return 0;
+  }
+  const bool need_cpy = W == REMIFA_R_32F && (c.I != REMIFA_FP32 || c.O != REMIFA_FP32);
+  const size_t ws = lu_workspace_bytes(algo, c, m, n, nb, d_cpy != nullptr);
+  RF_CHECK(ensure_workspace(h, ws));
+  Arena ar{(char*)h->ws};
+  Ctx x{h, c, c.ib, nullptr};
+  x.dsnap = ar.take(32 * 32 * 8);
+  void* cpy = d_cpy;
+  int64_t ldc = ldcpy;
+  if (need_cpy && !cpy) {
+    cpy = ar.take((size_t)m * n * 2);
+    ldc = m;
+  }
+  const bool buffered = algo == REMIFA_LL && W == REMIFA_R_16F && c.IP == REMIFA_R_32F;
+  void *pl = nullptr, *pu = nullptr;
+  int64_t ldpl = m, ldpu = nb;
+  if (buffered) {
+    pl = ar.take((size_t)m * nb * 4);
+    pu = ar.take((size_t)nb * m * 4);
+  }
+  h->ws_used = ar.off;
+  int st = 0;
+  const int64_t nc = (n - 1) / nb + 1;
+
+  auto run = [&]() -> int {
+    if (algo == REMIFA_LL) {
+      for (int64_t kk = 0; kk < nc; ++kk) {
+        const int64_t ofs = kk * nb;
+        const int64_t in = std::min(n - ofs, nb), updm = m - ofs;
+        void* blk = at(d_a, W, ofs, ofs, ldda);
+        // operands of the outer update: fp16 copy (W = fp32, TC) or A itself
+        const bool o_copy = W == REMIFA_R_32F && c.O != REMIFA_FP32;
+        const void* la = o_copy ? at(cpy, REMIFA_R_16F, ofs, 0, ldc) : at(d_a, W, ofs, 0, ldda);
+        const void* ua = o_copy ? at(cpy, REMIFA_R_16F, 0, ofs, ldc) : at(d_a, W, 0, ofs, ldda);
+        const void* ua2 = o_copy ? at(cpy, REMIFA_R_16F, 0, ofs + in, ldc)
+                                 : at(d_a, W, 0, ofs + in, ldda);
+        const int64_t lo = o_copy ? ldc : ldda;
+        if (!buffered) {
+          if (ofs > 0) {
+            // block column (:239-304) and block row (:311-410), in place
+            RF_CHECK(gemm(x, c.O, W, updm, in, ofs, la, lo, ua, lo, blk, ldda));
+            if (updm > in)
+              RF_CHECK(gemm(x, c.O, W, in, updm - in, ofs, la, lo, ua2, lo,
+                            at(d_a, W, ofs, ofs + in, ldda), ldda));
+          }
+          void* cblk = need_cpy ? at(cpy, REMIFA_R_16F, ofs, ofs, ldc) : nullptr;
+          RF_CHECK(inner_ll(x, W, updm, in, blk, ldda, blk, ldda, cblk, ldc, false));
+          // fp16 copy of the new factors for later TC outer updates (:554-563)
+          if (need_cpy && c.I == REMIFA_FP32) {
+            if (updm > in)
+              RF_CHECK(convert_launch(h, W, REMIFA_R_16F, in, updm - in,
+                                      at(d_a, W, ofs, ofs + in, ldda), ldda,
+                                      at(cpy, REMIFA_R_16F, ofs, ofs + in, ldc), ldc));
+            RF_CHECK(convert_launch(h, W, REMIFA_R_16F, updm, in, blk, ldda,
+                                    at(cpy, REMIFA_R_16F, ofs, ofs, ldc), ldc));
+          }
+        } else {
+          // fp16 storage, fp32 inner buffers: pl = block column (updm x in),
+          // pu = block row incl. the diagonal block (in x updm)
+          if (c.OP == REMIFA_R_16F && ofs > 0) {
+            RF_CHECK(gemm(x, c.O, W, updm, in, ofs, la, lo, ua, lo, blk, ldda));
+            if (updm > in)
+              RF_CHECK(gemm(x, c.O, W, in, updm - in, ofs, la, lo, ua2, lo,
+                            at(d_a, W, ofs, ofs + in, ldda), ldda));
+          }
+          RF_CHECK(convert_launch(h, W, REMIFA_R_32F, updm, in, blk, ldda, pl, ldpl));
+          RF_CHECK(convert_launch(h, W, REMIFA_R_32F, in, updm, blk, ldda, pu, ldpu));
+          if (c.OP == REMIFA_R_32F && ofs > 0) {
+            RF_CHECK(gemm(x, c.O, REMIFA_R_32F, updm, in, ofs, la, lo, ua, lo, pl, ldpl));
+            if (updm > in)
+              RF_CHECK(gemm(x, c.O, REMIFA_R_32F, in, updm, ofs, la, lo, ua, lo, pu, ldpu));
+          }
+          // inner factorization in fp32; the TC operands are the fp16 factors
+          // already written to A (the reference passes d_a as its fp16 copy,
+          // src/lu_nopiv_ll.cxx:632)
+          RF_CHECK(inner_ll(x, REMIFA_R_32F, updm, in, pl, ldpl, pu, ldpu, blk, ldda, true));
+        }
+      }
+    } else {
+      // right-looking
+      const bool tc_in = W == REMIFA_R_32F && c.I != REMIFA_FP32;
+      const bool tc_out = W == REMIFA_R_32F && c.O != REMIFA_FP32;
+      for (int64_t kk = 0; kk < nc; ++kk) {
+        const int64_t ofs = kk * nb, ofst = (kk + 1) * nb;
+        const int64_t in = std::min(n - ofs, nb), updm = m - ofs;
+        void* p = at(d_a, W, ofs, ofs, ldda);
+        const int64_t inc = (in - 1) / x.ib + 1;
+        for (int64_t k = 0; k < inc; ++k) {
+          const int64_t iofs = k * x.ib, cblkm = updm - iofs;
+          const int64_t cblkn = std::min(in - iofs, x.ib);
+          const int64_t iofst = (k + 1) * x.ib;
+          const int64_t tblkm = updm - iofst, tblkn = in - iofst;
+          void* b = at(p, W, iofs, iofs, ldda);
+          RF_CHECK(copy2d_launch(h, dtype_size(W), cblkn, cblkn, b, ldda, x.dsnap, x.ib));
+          RF_CHECK(panel_launch(h, W, (int)x.ib, cblkm, cblkn, x.dsnap, x.ib, b, ldda, b, ldda,
+                                nullptr));
+          if (tc_in || tc_out) {
+            // fp16 copy of the new ib column and row (:131-137)
+            if (cblkm > cblkn)
+              RF_CHECK(convert_launch(h, W, REMIFA_R_16F, cblkn, cblkm - cblkn,
+                                      at(p, W, iofs, iofs + cblkn, ldda), ldda,
+                                      at(cpy, REMIFA_R_16F, ofs + iofs, ofs + iofs + cblkn, ldc),
+                                      ldc));
+            RF_CHECK(convert_launch(h, W, REMIFA_R_16F, cblkm, cblkn, b, ldda,
+                                    at(cpy, REMIFA_R_16F, ofs + iofs, ofs + iofs, ldc), ldc));
+          }
+          if (tblkn > 0) {
+            const void* lop = tc_in ? at(cpy, REMIFA_R_16F, ofs + iofst, ofs + iofs, ldc)
+                                    : at(p, W, iofst, iofs, ldda);
+            const void* uop = tc_in ? at(cpy, REMIFA_R_16F, ofs + iofs, ofs + iofst, ldc)
+                                    : at(p, W, iofs, iofst, ldda);
+            const int64_t lop_ld = tc_in ? ldc : ldda;
+            // rank-ib update of the outer block column (:139-163)
+            RF_CHECK(gemm(x, c.I, W, tblkm, tblkn, x.ib, lop, lop_ld, uop, lop_ld,
+                          at(p, W, iofst, iofst, ldda), ldda));
+            // and of the outer block row beyond the outer block (:167-193)
+            if (tblkm > tblkn) {
+              const void* uop2 = tc_in ? at(cpy, REMIFA_R_16F, ofs + iofs, ofst, ldc)
+                                       : at(p, W, iofs, nb, ldda);
+              RF_CHECK(gemm(x, c.I, W, tblkn, tblkm - tblkn, x.ib, lop, lop_ld, uop2, lop_ld,
+                            at(p, W, iofst, nb, ldda), ldda));
+            }
+          }
+        }
+        // Schur complement (:170-195)
+        const int64_t tm = m - ofst, tn = n - ofst;
+        if (tn > 0) {
+          const void* lop = tc_out ? at(cpy, REMIFA_R_16F, ofst, ofs, ldc) : at(d_a, W, ofst, ofs, ldda);
+          const void* uop = tc_out ? at(cpy, REMIFA_R_16F, ofs, ofst, ldc) : at(d_a, W, ofs, ofst, ldda);
+          const int64_t ldo = tc_out ? ldc : ldda;
+          RF_CHECK(gemm(x, c.O, W, tm, tn, nb, lop, ldo, uop, ldo,
+                        at(d_a, W, ofst, ofst, ldda), ldda));
+        }
+      }
+    }
+    return 0;
+  };
+  st = run();
+  h->ws_used = 0;
+  if (st) return st;
+  if (d_info) {
+    info_kernel<<<1, 256, 0, h->stream>>>(n, d_a, ldda, W, d_info);
+    RF_HIP(hipGetLastError());
+  }
+  // the reference drivers synchronise before returning (:567-569)
+  RF_HIP(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+}  // namespace remifa
