@@ -1,0 +1,239 @@
+#!/usr/bin/env python3
+"""Benchmark: no-pivot LU at n=32768 on one MI355X, FP16-MFMA + FP64-residual
+iterative refinement vs plain FP32.
+
+One step = the hot path on one matrix: factor the fp16 (half-footprint) copy
+of A with MFMA updates (fp16 operands, fp32 accumulation), then refine the
+solution of A x = b with the FP64 residual until the normwise backward error
+(tests/errors.hxx:186-249) is <= 1e-14. A (fp32, seeded, alpha = 1.05) and b
+are generated on the GPU and stay resident; restoring the fp16 working copy
+before each step is outside the timed region, as the reference's harness
+converts before starting its timer (tests/testing_lu_gpu.hxx:514, 531).
+
+Printed: ONE JSON line. value = LU TFLOP/s of the fp16 factorization
+(2n^3/3 / t_factor, tests/testing_lu_gpu.hxx:1412-1414); ms_per_step = time to
+solution (factor + refinement). The plain FP32 GPU factorization is timed in
+the same run (with an FP32 solve and with the same FP64 refinement), and the
+oracle's CPU restatement of lu_nopiv_ll_f32 on a bounded sample gives
+cpu_baseline. The roofline figure of the dominant kernel (the MFMA update
+GEMM) is computed from hipEvents recorded around every GEMM launch on the
+library's stream during the timed steps.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "LU TFLOP/s & time to 1e-14 solve at n=32k: FP16-MFMA+IR vs plain FP32"
+PEAK_FP16_TFLOPS = 2500.0  # MI355X dense fp16/bf16 matrix peak
+PEAK_FP32_TFLOPS = 157.3
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--n", type=int, default=32768)
+    p.add_argument("--nb", type=int, default=512)
+    p.add_argument("--ib", type=int, default=32)
+    p.add_argument("--algo", choices=["ll", "rl"], default="ll")
+    p.add_argument("--alpha", type=float, default=1.05)
+    p.add_argument("--seed", type=int, default=2024)
+    p.add_argument("--nrhs", type=int, default=1)
+    p.add_argument("--fp32-steps", type=int, default=1)
+    p.add_argument("--fp32-nb", type=int, default=0)
+    p.add_argument("--cpu-n", type=int, default=0, help="CPU baseline sample size (0 = auto)")
+    p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-fp32", action="store_true")
+    return p.parse_args()
+
+
+def cpu_baseline(target_s: float, n_fixed: int):
+    """Oracle restatement of lu_nopiv_ll_f32 (nb=256, ib=8) on the host
+    cores; the sample size is grown until one factorization takes a few
+    seconds, then timed once."""
+    import numpy as np
+    import oracle as orc
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    orc.set_threads(threads)
+    c = orc.cfg()
+    n = n_fixed or 1024
+    while True:
+        a = orc.gen_diagdom_alpha(n, 1, 1.5)
+        t0 = time.perf_counter()
+        orc.lu_nopiv("ll", c, a, 256)
+        dt = time.perf_counter() - t0
+        if n_fixed or dt > target_s / 8 or n >= 16384:
+            break
+        n *= 2
+    tflops = (2.0 * n**3 / 3.0) / dt / 1e12
+    return {"value": tflops, "unit": "TFLOP/s", "cores": orc.get_threads(), "kind": "port",
+            "sample": f"oracle LL restatement of lu_nopiv_ll_f32 (nb=256, ib=8), n={n}, "
+                      f"one factorization, {dt:.2f} s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU "
+              "design is described in DESIGN.md and not built.", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import remifa_amd as rf
+
+    dev = torch.device("cuda:0")
+    n, nb = a.n, a.nb
+    h = rf.Handle()
+    s = h.stream
+    A = rf.colmajor_empty(n, n, device=dev)
+    rf.gen_diagdom(h, A, seed=a.seed, alpha=a.alpha)
+    bf = rf.colmajor_empty(n, a.nrhs, device=dev)
+    rf.gen_uniform(h, bf, seed=a.seed + 1)
+    B = torch.zeros((a.nrhs, n), dtype=torch.float64, device=dev).t()
+    B.copy_(bf)
+    X = torch.zeros((a.nrhs, n), dtype=torch.float64, device=dev).t()
+    A16 = rf.colmajor_empty(n, n, dtype=torch.float16, device=dev)
+    algo = rf.LL if a.algo == "ll" else rf.RL
+    if algo == rf.LL:
+        cfg16 = rf.api.make_config(W=rf.R_16F, I=rf.TC32, O=rf.TC32, OP=rf.R_32F, ib=a.ib)
+    else:
+        cfg16 = rf.api.make_config(W=rf.R_16F, I=rf.TC32, O=rf.TC32, ib=a.ib)
+    info = torch.zeros(1, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+
+    def ev():
+        e = torch.cuda.Event(enable_timing=True)
+        return e
+
+    def step16(timing: bool):
+        rf.convert(h, A, A16)  # restore the working copy (untimed)
+        X.zero_()
+        e0, e1, e2 = ev(), ev(), ev()
+        if timing:
+            h.timing(True)
+        e0.record(s)
+        rf.lu_nopiv(h, A16, nb, algo=algo, cfg=cfg16, info=info)
+        e1.record(s)
+        it, be, hist = rf.ir_solve(h, A, A16, B, X, tol=1e-14, maxit=30)
+        e2.record(s)
+        h.timing(False)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1), e1.elapsed_time(e2), it, be
+
+    for _ in range(a.warmup):
+        step16(False)
+    h.timing_reset()
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    res = [step16(True) for _ in range(a.steps)]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    t_fac = sum(r[0] for r in res) / len(res)
+    t_ir = sum(r[1] for r in res) / len(res)
+    iters, nwerr = res[-1][2], res[-1][3]
+    gemm_ms, gemm_n, gemm_flops = h.timing_query(0)
+    panel_ms, panel_n, panel_bytes = h.timing_query(1)
+    conv_ms, conv_n, conv_bytes = h.timing_query(2)
+    trsm_ms, trsm_n, _ = h.timing_query(3)
+    res_ms, res_n, _ = h.timing_query(4)
+    h.timing_reset()
+    # one more step without per-kernel events, to show their overhead
+    t_fac_plain, t_ir_plain, _, _ = step16(False)
+
+    flops = 2.0 * n**3 / 3.0
+    gemm_tflops = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
+    out = {
+        "metric": METRIC,
+        "value": round(flops / (t_fac * 1e-3) / 1e12, 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(t_fac + t_ir, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp16 (fp32 accumulate); fp64 residual",
+        "data": f"synthetic: seeded minstd_rand0 uniform(-1,1), a_ii = {a.alpha}*sum|a_ij|, "
+                f"generated on the GPU",
+        "config": {"workload": f"no-pivot LU n={n} fp16 storage, TC32 MFMA updates, "
+                               f"+ FP64-residual IR to 1e-14, nrhs={a.nrhs}",
+                   "n": n, "nb": nb, "ib": a.ib, "algo": a.algo, "alpha": a.alpha,
+                   "nrhs": a.nrhs, "seed": a.seed,
+                   "precision": "W=f16 I=O=TC32 OP=IP=OF=IF=f32" if algo == rf.LL
+                   else "W=f16 I=O=TC32"},
+        "factor_ms": round(t_fac, 3),
+        "ir_ms": round(t_ir, 3),
+        "ir_iters": iters,
+        "backward_error": nwerr,
+        "wall_s_timed_steps": round(wall, 3),
+        "factor_ms_without_kernel_events": round(t_fac_plain, 3),
+        "kernels": {
+            "gemm": {"ms": round(gemm_ms / a.steps, 3), "launches": gemm_n // a.steps,
+                     "tflops": round(gemm_tflops, 2)},
+            "panel": {"ms": round(panel_ms / a.steps, 3), "launches": panel_n // a.steps,
+                      "gbps": round(panel_bytes / (panel_ms * 1e-3) / 1e9, 1) if panel_ms else 0},
+            "convert": {"ms": round(conv_ms / a.steps, 3), "launches": conv_n // a.steps,
+                        "gbps": round(conv_bytes / (conv_ms * 1e-3) / 1e9, 1) if conv_ms else 0},
+            "trsm": {"ms": round(trsm_ms / a.steps, 3), "launches": trsm_n // a.steps},
+            "residual": {"ms": round(res_ms / a.steps, 3), "launches": res_n // a.steps},
+        },
+        "roofline": {"bound": "mfma", "achieved": round(gemm_tflops, 2),
+                     "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(gemm_tflops / PEAK_FP16_TFLOPS, 4), "traffic": None,
+                     "kernel": "gemm_mfma_kernel (update GEMMs, fp16 operands)",
+                     "work_per_launch": "2*M*N*K flops of the launch's update"},
+    }
+
+    if not a.no_fp32:
+        # plain FP32 GPU factorization (W=f32, I=O=FP32), same matrix
+        A32 = rf.colmajor_empty(n, n, device=dev)
+        cfg32 = rf.api.make_config(W=rf.R_32F, I=rf.FP32, O=rf.FP32, ib=a.ib)
+        nb32 = a.fp32_nb or nb
+        bsol = rf.colmajor_empty(n, a.nrhs, device=dev)
+        t32 = []
+        for k in range(a.fp32_steps + 1):
+            A32.copy_(A)
+            torch.cuda.synchronize()
+            e0, e1, e2, e3 = ev(), ev(), ev(), ev()
+            e0.record(s)
+            rf.lu_nopiv(h, A32, nb32, algo=algo, cfg=cfg32, info=info)
+            e1.record(s)
+            bsol.copy_(bf)
+            rf.getrs_nopiv(h, A32, bsol)
+            e2.record(s)
+            X.zero_()
+            it32, be32, _ = rf.ir_solve(h, A, A32, B, X, tol=1e-14, maxit=30)
+            e3.record(s)
+            torch.cuda.synchronize()
+            if k > 0 or a.fp32_steps == 0:
+                t32.append((e0.elapsed_time(e1), e1.elapsed_time(e2), e2.elapsed_time(e3), it32))
+        lu32 = sum(t[0] for t in t32) / len(t32)
+        sv32 = sum(t[1] for t in t32) / len(t32)
+        ir32 = sum(t[2] for t in t32) / len(t32)
+        tts = t_fac + t_ir
+        out["fp32"] = {"factor_ms": round(lu32, 3), "tflops": round(flops / (lu32 * 1e-3) / 1e12, 3),
+                       "solve_ms": round(sv32, 3), "ir_ms": round(ir32, 3),
+                       "ir_iters": t32[-1][3], "nb": nb32}
+        out["speedup_tts_vs_fp32_lu_fp32_solve"] = round((lu32 + sv32) / tts, 3)
+        out["speedup_tts_vs_fp32_lu_fp64_ir"] = round((lu32 + ir32) / tts, 3)
+        out["speedup_factor_vs_fp32"] = round(lu32 / t_fac, 3)
+        del A32
+
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(a.cpu_seconds, a.cpu_n)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -67,7 +67,9 @@ __global__ void __launch_bounds__(256) trsm_kernel(int64_t n, int nrhs, const LT
       int a = e % NBS, bq = e / NBS;
       int rr = TRANS ? bq : a, cc = TRANS ? a : bq;
       int64_t R = r0 + rr, Cc = c0 + cc;
-      Ts[rr][cc] = (R < n && Cc < n) ? tel<LT, TRANS>(lu, ld, R, Cc) : 0.0f;
+      // rows/columns past n are padded with the identity so that the
+      // substitution leaves them at zero (no 0/0 in a ragged last block)
+      Ts[rr][cc] = (R < n && Cc < n) ? tel<LT, TRANS>(lu, ld, R, Cc) : (R == Cc ? 1.0f : 0.0f);
     }
   };
   // off-diagonal blocks, in dependency order

@@ -21,7 +21,7 @@ def _rf():
     return rf
 
 
-@pytest.mark.parametrize("n,nrhs,lu_t", [(1, 1, "f32"), (63, 1, "f32"), (64, 3, "f32"),
+@pytest.mark.parametrize("n,nrhs,lu_t", [(2, 1, "f32"), (63, 1, "f32"), (64, 3, "f32"),
                                          (300, 16, "f32"), (513, 20, "f16"), (1000, 1, "f16")])
 @pytest.mark.parametrize("trans", [False, True])
 def test_getrs_matches_f64(handle, gpu, n, nrhs, lu_t, trans):
