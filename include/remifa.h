@@ -101,6 +101,11 @@ int remifa_timing_enable(remifa_handle_t handle, int enable);
 int remifa_timing_query(remifa_handle_t handle, int cls, double *ms, int64_t *launches,
                         double *work);
 int remifa_timing_reset(remifa_handle_t handle);
+/* Every recorded launch (up to max): class, kernel variant (for GEMMs:
+ * operand kind << 8 | fp16-output << 4 | tile shape, see remifa_amd/csrc/
+ * gemm.hip), elapsed ms and work. *count receives the number recorded. */
+int remifa_timing_records(remifa_handle_t handle, int64_t max, int *cls, int *variant, double *ms,
+                          double *work, int64_t *count);
 
 /* ------------------------------------------------------------ drivers */
 /* Generic no-pivot LU of the m x n (m == n) column-major matrix d_a.

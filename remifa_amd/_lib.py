@@ -52,6 +52,7 @@ SIGNATURES = {
     "remifa_timing_enable": [_vp, _i],
     "remifa_timing_query": [_vp, _i, _pd, _pi64, _pd],
     "remifa_timing_reset": [_vp],
+    "remifa_timing_records": [_vp, _i64, _vp, _vp, _vp, _vp, _pi64],
     "remifa_lu_nopiv": [_vp, _i, C.POINTER(LuConfig), _i64, _i64, _i64, _vp, _i64, _vp, _vp, _i64],
     "remifa_lu_nopiv_ll_f32": [_vp, _i, _i, _i, _vp, _i, _vp],
     "remifa_lu_nopiv_ll_f32_fp32xtc32": [_vp, _i, _i, _i, _vp, _i, _vp, _vp, _i],

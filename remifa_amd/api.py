@@ -106,6 +106,34 @@ class Handle:
               "remifa_timing_query")
         return ms.value, n.value, w.value
 
+    def timing_records(self):
+        """(cls, variant, ms, work) numpy arrays, one entry per recorded launch."""
+        import numpy as np
+        cnt = C.c_int64()
+        check(lib().remifa_timing_records(self._h, 0, None, None, None, None, C.byref(cnt)),
+              "remifa_timing_records")
+        k = cnt.value
+        cls, var = np.zeros(k, np.int32), np.zeros(k, np.int32)
+        ms, work = np.zeros(k, np.float64), np.zeros(k, np.float64)
+        vp = lambda a: C.c_void_p(a.ctypes.data)
+        check(lib().remifa_timing_records(self._h, k, vp(cls), vp(var), vp(ms), vp(work),
+                                          C.byref(cnt)), "remifa_timing_records")
+        return cls, var, ms, work
+
+
+# GEMM variant codes of the timer: opk << 8 | fp16-output << 4 | tile
+_GEMM_OPK = {0: "fp16", 1: "bf16", 2: "fp32"}
+_GEMM_TILE = {0: (256, 32, 4, 1), 1: (32, 256, 1, 4), 2: (256, 64, 4, 1), 3: (64, 256, 1, 4),
+              4: (256, 256, 2, 2), 5: (128, 128, 2, 2)}
+
+
+def gemm_variant_name(v: int) -> str:
+    """Template instantiation of gemm_mfma_kernel for a GEMM variant code."""
+    opk, out16, tile = v >> 8, (v >> 4) & 15, v & 15
+    bm, bn, wm, wn = _GEMM_TILE.get(tile, (0, 0, 0, 0))
+    return (f"gemm_mfma_kernel<{opk}, {'_Float16' if out16 else 'float'}, {bm}, {bn}, {wm}, {wn}>"
+            f" ({_GEMM_OPK.get(opk, '?')} operands, {bm}x{bn} tile)")
+
 
 def _h(handle) -> C.c_void_p:
     return handle.value if isinstance(handle, Handle) else handle

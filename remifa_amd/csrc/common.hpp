@@ -51,6 +51,7 @@ struct KernelTimer {
     int cls;
     int ev0, ev1;
     double work;  // algorithmic flops or bytes of this launch
+    int variant;  // kernel instantiation (tile shape, operand kind, ...)
   };
   std::vector<Rec> recs;
   size_t next = 0;
@@ -87,7 +88,7 @@ enum KernelClass {
   KC_OTHER = 5,
   KC_COUNT = 6
 };
-void timer_begin(remifa_handle_t h, int cls, double work);
+void timer_begin(remifa_handle_t h, int cls, double work, int variant = 0);
 void timer_end(remifa_handle_t h);
 
 // ---------------------------------------------------------------- device conversions

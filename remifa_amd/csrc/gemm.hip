@@ -382,24 +382,37 @@ int launch_mfma(remifa_handle_t h, GemmArgs g, int split, bool vec) {
   return 0;
 }
 
+// tile shapes (BM x BN); the timer's variant code of a launch is
+// opk << 8 | fp16-output << 4 | tile
+enum Tile { T256x32 = 0, T32x256 = 1, T256x64 = 2, T64x256 = 3, T256x256 = 4, T128x128 = 5 };
+
+// 256x256 for large outputs; narrow shapes (N or M = ib .. 64) use 256x32 /
+// 256x64 and their transposes so that skinny updates keep the chip busy.
+int choose_tile(int opk, int64_t M, int64_t N) {
+  if (N <= 32) return T256x32;
+  if (M <= 32) return T32x256;
+  if (opk == OPK_F32) return T128x128;
+  if (N <= 64) return T256x64;
+  if (M <= 64) return T64x256;
+  if (M * N >= (int64_t)256 * 256 * 512) return T256x256;
+  return T128x128;
+}
+
 template <int OPK, typename OutT>
-int dispatch_tiles(remifa_handle_t h, GemmArgs g, int split, bool vec) {
-  // tile choice: 256x256 for large outputs; narrow shapes use 256x64 /
-  // 64x256 so that skinny updates (N or M = ib .. nb) keep the chip busy.
-  const int64_t M = g.M, N = g.N;
-  if constexpr (OPK == OPK_F32) {
-    if (N <= 32) return launch_mfma<OPK, OutT, 256, 32, 4, 1>(h, g, split, vec);
-    if (M <= 32) return launch_mfma<OPK, OutT, 32, 256, 1, 4>(h, g, split, vec);
-    return launch_mfma<OPK, OutT, 128, 128, 2, 2>(h, g, split, vec);
-  } else {
-    if (N <= 32) return launch_mfma<OPK, OutT, 256, 32, 4, 1>(h, g, split, vec);
-    if (M <= 32) return launch_mfma<OPK, OutT, 32, 256, 1, 4>(h, g, split, vec);
-    if (N <= 64) return launch_mfma<OPK, OutT, 256, 64, 4, 1>(h, g, split, vec);
-    if (M <= 64) return launch_mfma<OPK, OutT, 64, 256, 1, 4>(h, g, split, vec);
-    if (M * N >= (int64_t)256 * 256 * 512)
-      return launch_mfma<OPK, OutT, 256, 256, 2, 2>(h, g, split, vec);
-    return launch_mfma<OPK, OutT, 128, 128, 2, 2>(h, g, split, vec);
+int dispatch_tiles(remifa_handle_t h, int tile, GemmArgs g, int split, bool vec) {
+  switch (tile) {
+    case T256x32: return launch_mfma<OPK, OutT, 256, 32, 4, 1>(h, g, split, vec);
+    case T32x256: return launch_mfma<OPK, OutT, 32, 256, 1, 4>(h, g, split, vec);
+    case T128x128: return launch_mfma<OPK, OutT, 128, 128, 2, 2>(h, g, split, vec);
   }
+  if constexpr (OPK != OPK_F32) {
+    switch (tile) {
+      case T256x64: return launch_mfma<OPK, OutT, 256, 64, 4, 1>(h, g, split, vec);
+      case T64x256: return launch_mfma<OPK, OutT, 64, 256, 1, 4>(h, g, split, vec);
+      case T256x256: return launch_mfma<OPK, OutT, 256, 256, 2, 2>(h, g, split, vec);
+    }
+  }
+  return REMIFA_ERR_UNSUPPORTED;
 }
 
 }  // namespace
@@ -482,17 +495,18 @@ int gemm_launch(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t m, in
     g.ldw = m;
   }
 
-  timer_begin(h, KC_GEMM, flops);
+  const int tile = choose_tile(opk, m, n);
+  timer_begin(h, KC_GEMM, flops, (opk << 8) | ((out_t == REMIFA_R_16F) << 4) | tile);
   int st;
   if (opk == OPK_F32) {
-    st = out_t == REMIFA_R_32F ? dispatch_tiles<OPK_F32, float>(h, g, split, vec)
-                               : dispatch_tiles<OPK_F32, half_t>(h, g, split, vec);
+    st = out_t == REMIFA_R_32F ? dispatch_tiles<OPK_F32, float>(h, tile, g, split, vec)
+                               : dispatch_tiles<OPK_F32, half_t>(h, tile, g, split, vec);
   } else if (opk == OPK_F16) {
-    st = out_t == REMIFA_R_32F ? dispatch_tiles<OPK_F16, float>(h, g, split, vec)
-                               : dispatch_tiles<OPK_F16, half_t>(h, g, split, vec);
+    st = out_t == REMIFA_R_32F ? dispatch_tiles<OPK_F16, float>(h, tile, g, split, vec)
+                               : dispatch_tiles<OPK_F16, half_t>(h, tile, g, split, vec);
   } else {
-    st = out_t == REMIFA_R_32F ? dispatch_tiles<OPK_BF16, float>(h, g, split, vec)
-                               : dispatch_tiles<OPK_BF16, half_t>(h, g, split, vec);
+    st = out_t == REMIFA_R_32F ? dispatch_tiles<OPK_BF16, float>(h, tile, g, split, vec)
+                               : dispatch_tiles<OPK_BF16, half_t>(h, tile, g, split, vec);
   }
   if (st == 0 && split > 1) {
     dim3 grid((unsigned)ceil_div(m, 256), (unsigned)(n < 65535 ? n : 65535));

@@ -50,12 +50,12 @@ static int timer_event(remifa_handle_t h) {
   return (int)t.next++;
 }
 
-void timer_begin(remifa_handle_t h, int cls, double work) {
+void timer_begin(remifa_handle_t h, int cls, double work, int variant) {
   if (!h->timer.enabled) return;
   int e0 = timer_event(h), e1 = timer_event(h);
   if (e0 < 0 || e1 < 0) return;
   hipEventRecord(h->timer.pool[e0], h->stream);
-  h->timer.recs.push_back({cls, e0, e1, work});
+  h->timer.recs.push_back({cls, e0, e1, work, variant});
 }
 
 void timer_end(remifa_handle_t h) {
@@ -164,6 +164,28 @@ int remifa_timing_query(remifa_handle_t h, int cls, double* ms, int64_t* launche
   if (ms) *ms = tot;
   if (launches) *launches = cnt;
   if (work) *work = w;
+  return 0;
+}
+
+int remifa_timing_records(remifa_handle_t h, int64_t max, int* cls, int* variant, double* ms,
+                          double* work, int64_t* count) {
+  if (!h) return -1;
+  if (max < 0) return -2;
+  if (!count) return -8;
+  RF_HIP(hipStreamSynchronize(h->stream));
+  int64_t k = 0;
+  for (auto& r : h->timer.recs) {
+    if (k < max) {
+      float t = 0.0f;
+      RF_HIP(hipEventElapsedTime(&t, h->timer.pool[r.ev0], h->timer.pool[r.ev1]));
+      if (cls) cls[k] = r.cls;
+      if (variant) variant[k] = r.variant;
+      if (ms) ms[k] = t;
+      if (work) work[k] = r.work;
+    }
+    ++k;
+  }
+  *count = k;
   return 0;
 }
 

@@ -88,14 +88,17 @@ __global__ void __launch_bounds__(256) trsm_kernel(int64_t n, int nrhs, const LT
       }
     }
     __syncthreads();
-    // every thread acquires the producer's release before reading X_j
-    (void)__hip_atomic_load(&sync[2 + bj], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    // X_j was published with agent-coherent stores before its flag; read it
+    // with agent-coherent loads (they bypass the non-coherent caches, so no
+    // cache invalidation is needed)
     {
       const int64_t c0 = (int64_t)bj * NBS;
       for (int e = t; e < NBS * RC; e += 256) {
         int c = e % NBS, k = e / NBS;
         int64_t C = c0 + c;
-        Xs[c][k] = (C < n && k < nrhs) ? b[C + k * ldb] : 0.0f;
+        Xs[c][k] = (C < n && k < nrhs)
+                       ? __hip_atomic_load(&b[C + k * ldb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                       : 0.0f;
       }
     }
     __syncthreads();
@@ -131,7 +134,8 @@ __global__ void __launch_bounds__(256) trsm_kernel(int64_t n, int nrhs, const LT
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     int k = kg * 4 + q;
-    if (gr < n && k < nrhs) b[gr + k * ldb] = acc[q];
+    if (gr < n && k < nrhs)
+      __hip_atomic_store(&b[gr + k * ldb], acc[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
