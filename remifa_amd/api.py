@@ -93,6 +93,9 @@ class Handle:
     def set_tuning(self, panel_width: int = 0, split_k: int = 0):
         check(lib().remifa_set_tuning(self._h, panel_width, split_k), "remifa_set_tuning")
 
+    def set_gemm_tile(self, tile: int = -1):
+        check(lib().remifa_set_gemm_tile(self._h, tile), "remifa_set_gemm_tile")
+
     # per-kernel timing -------------------------------------------------
     def timing(self, enable: bool = True):
         check(lib().remifa_timing_enable(self._h, int(enable)), "remifa_timing_enable")

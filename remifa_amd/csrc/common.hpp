@@ -72,6 +72,7 @@ struct remifa_handle_st {
   // tuning knobs (0 = automatic)
   int panel_width = 0;
   int split_k = 0;
+  int gemm_tile = -1;  // -1: chosen per shape
 };
 
 namespace remifa {

@@ -26,276 +26,7 @@
 namespace remifa {
 namespace {
 
-typedef _Float16 h8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
-typedef float f16v __attribute__((ext_vector_type(16)));
-typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-enum { OPK_F16 = 0, OPK_BF16 = 1, OPK_F32 = 2 };
-
-struct GemmArgs {
-  int64_t M, N, K;
-  int64_t lda, ldb, ldc;
-  float alpha, beta;
-  const void* A;
-  const void* B;
-  void* C;
-  float* W;        // split-K partials [split][N][ldw]
-  int64_t ldw;
-  int64_t kchunk;  // K range per split (multiple of BK)
-  int tiles_m, tiles_n;
-};
-
-// --- per operand kind traits --------------------------------------------
-template <int OPK>
-struct OpTraits;
-template <>
-struct OpTraits<OPK_F16> {
-  using Store = half_t;     // storage type in HBM
-  using Lds = uint16_t;     // element in LDS
-  static constexpr int BK = 32;
-  static constexpr int CH = 8;  // elements per 16-byte chunk
-  using Chunk = u16x8;
-  static __device__ __forceinline__ Chunk cvt(Chunk c) { return c; }
-};
-template <>
-struct OpTraits<OPK_BF16> {
-  using Store = half_t;
-  using Lds = uint16_t;
-  static constexpr int BK = 32;
-  static constexpr int CH = 8;
-  using Chunk = u16x8;
-  // fp16 storage -> bf16 operand, round to nearest even
-  static __device__ __forceinline__ Chunk cvt(Chunk c) {
-    Chunk r;
-#pragma unroll
-    for (int v = 0; v < 8; ++v) {
-      float f = (float)__builtin_bit_cast(half_t, (uint16_t)c[v]);
-      r[v] = f32_to_bf16_bits(f);
-    }
-    return r;
-  }
-};
-template <>
-struct OpTraits<OPK_F32> {
-  using Store = float;
-  using Lds = float;
-  static constexpr int BK = 16;
-  static constexpr int CH = 4;
-  using Chunk = f32x4;
-  static __device__ __forceinline__ Chunk cvt(Chunk c) { return c; }
-};
-
-// XCD-aware tile index: workgroup w runs on XCD w % 8; give every XCD a
-// contiguous range of tiles (a bijection for any tile count).
-__device__ __forceinline__ int xcd_tile(int w, int nt) {
-  int q = nt >> 3, r = nt & 7;
-  int x = w & 7, s = w >> 3;
-  return x * q + (x < r ? x : r) + s;
-}
-
-template <typename OutT>
-__device__ __forceinline__ float ld_out(const OutT* p) {
-  return (float)*p;
-}
-
-template <int OPK, typename OutT, int BM, int BN, int WM, int WN, bool VEC>
-__global__ void __launch_bounds__(256) gemm_mfma_kernel(GemmArgs g) {
-  using TR = OpTraits<OPK>;
-  using S = typename TR::Store;
-  using L = typename TR::Lds;
-  using Chunk = typename TR::Chunk;
-  constexpr int BK = TR::BK;
-  constexpr int CH = TR::CH;
-  constexpr int WTM = BM / WM, WTN = BN / WN;
-  constexpr int FM = WTM / 32, FN = WTN / 32;
-  constexpr int PADA = (OPK == OPK_F32) ? 32 : 8;
-  constexpr int PADB = (OPK == OPK_F32) ? 4 : 8;
-  constexpr int SA = BM + PADA;  // As[k][m]
-  constexpr int SB = BK + PADB;  // Bs[n][k]
-  constexpr int TA = BK * BM / CH, TB = BN * BK / CH;  // 16-byte chunks per slab
-  constexpr int NA = (TA + 255) / 256;
-  constexpr int NB = (TB + 255) / 256;
-  static_assert(TA % 64 == 0 && TB % 64 == 0, "slab chunks must fill whole waves");
-  static_assert(FM >= 1 && FN >= 1, "wave tile below 32x32");
-
-  __shared__ __attribute__((aligned(16))) L As[2][BK * SA];
-  __shared__ __attribute__((aligned(16))) L Bs[2][BN * SB];
-
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int wm = wave % WM, wn = wave / WM;
-
-  const int nt = g.tiles_m * g.tiles_n;
-  const int tile = xcd_tile(blockIdx.x, nt);
-  const int tm = tile % g.tiles_m, tn = tile / g.tiles_m;
-  const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
-  const int64_t kbeg = (int64_t)blockIdx.z * g.kchunk;
-  int64_t kend = kbeg + g.kchunk;
-  if (kend > g.K) kend = g.K;
-  const int nk = (int)((kend - kbeg + BK - 1) / BK);
-
-  const S* __restrict__ A = (const S*)g.A;
-  const S* __restrict__ B = (const S*)g.B;
-
-  Chunk ra[NA], rb[NB];
-
-  auto load_tiles = [&](int kt) {
-    const int64_t kk0 = kbeg + (int64_t)kt * BK;
-#pragma unroll
-    for (int p = 0; p < NA; ++p) {
-      int c = tid + 256 * p;
-      if (TA % 256 && c >= TA) break;
-      int mq = c % (BM / CH), k = c / (BM / CH);
-      int64_t gm = m0 + mq * CH, gk = kk0 + k;
-      Chunk v;
-      const S* src = A + gm + gk * g.lda;
-      if (VEC && gk < kend && gm + CH <= g.M) {
-        v = *(const Chunk*)src;
-      } else {
-#pragma unroll
-        for (int e = 0; e < CH; ++e) {
-          S x = (gk < kend && gm + e < g.M) ? src[e] : (S)0;
-          if constexpr (OPK == OPK_F32)
-            v[e] = x;
-          else
-            v[e] = __builtin_bit_cast(uint16_t, x);
-        }
-      }
-      ra[p] = v;
-    }
-#pragma unroll
-    for (int p = 0; p < NB; ++p) {
-      int c = tid + 256 * p;
-      if (TB % 256 && c >= TB) break;
-      int kq = c % (BK / CH), n = c / (BK / CH);
-      int64_t gk = kk0 + kq * CH, gn = n0 + n;
-      Chunk v;
-      const S* src = B + gk + gn * g.ldb;
-      if (VEC && gn < g.N && gk + CH <= kend) {
-        v = *(const Chunk*)src;
-      } else {
-#pragma unroll
-        for (int e = 0; e < CH; ++e) {
-          S x = (gn < g.N && gk + e < kend) ? src[e] : (S)0;
-          if constexpr (OPK == OPK_F32)
-            v[e] = x;
-          else
-            v[e] = __builtin_bit_cast(uint16_t, x);
-        }
-      }
-      rb[p] = v;
-    }
-  };
-
-  auto store_tiles = [&](int buf) {
-#pragma unroll
-    for (int p = 0; p < NA; ++p) {
-      int c = tid + 256 * p;
-      if (TA % 256 && c >= TA) break;
-      int mq = c % (BM / CH), k = c / (BM / CH);
-      *(Chunk*)&As[buf][k * SA + mq * CH] = TR::cvt(ra[p]);
-    }
-#pragma unroll
-    for (int p = 0; p < NB; ++p) {
-      int c = tid + 256 * p;
-      if (TB % 256 && c >= TB) break;
-      int kq = c % (BK / CH), n = c / (BK / CH);
-      *(Chunk*)&Bs[buf][n * SB + kq * CH] = TR::cvt(rb[p]);
-    }
-  };
-
-  f16v acc[FM][FN];
-#pragma unroll
-  for (int i = 0; i < FM; ++i)
-#pragma unroll
-    for (int j = 0; j < FN; ++j)
-#pragma unroll
-      for (int q = 0; q < 16; ++q) acc[i][j][q] = 0.0f;
-
-  if (nk > 0) {
-    load_tiles(0);
-    store_tiles(0);
-  }
-  __syncthreads();
-
-  const int l32 = lane & 31, lh = lane >> 5;
-  for (int kt = 0; kt < nk; ++kt) {
-    const int buf = kt & 1;
-    if (kt + 1 < nk) load_tiles(kt + 1);
-    const L* as = As[buf];
-    const L* bs = Bs[buf];
-    if constexpr (OPK == OPK_F32) {
-#pragma unroll
-      for (int ks = 0; ks < BK; ks += 2) {
-        float xb[FN], ya[FM];
-#pragma unroll
-        for (int j = 0; j < FN; ++j) xb[j] = bs[(wn * WTN + j * 32 + l32) * SB + ks + lh];
-#pragma unroll
-        for (int i = 0; i < FM; ++i) ya[i] = as[(ks + lh) * SA + wm * WTM + i * 32 + l32];
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-          for (int j = 0; j < FN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(xb[j], ya[i], acc[i][j], 0, 0, 0);
-      }
-    } else {
-#pragma unroll
-      for (int ks = 0; ks < BK; ks += 16) {
-        u16x8 xb[FN], ya[FM];
-#pragma unroll
-        for (int j = 0; j < FN; ++j)
-          xb[j] = *(const u16x8*)&bs[(wn * WTN + j * 32 + l32) * SB + ks + 8 * lh];
-#pragma unroll
-        for (int i = 0; i < FM; ++i) {
-#pragma unroll
-          for (int v = 0; v < 8; ++v)
-            ya[i][v] = as[(ks + 8 * lh + v) * SA + wm * WTM + i * 32 + l32];
-        }
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-          for (int j = 0; j < FN; ++j) {
-            if constexpr (OPK == OPK_F16)
-              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
-                  __builtin_bit_cast(h8, xb[j]), __builtin_bit_cast(h8, ya[i]), acc[i][j], 0, 0, 0);
-            else
-              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                  __builtin_bit_cast(bf8, xb[j]), __builtin_bit_cast(bf8, ya[i]), acc[i][j], 0, 0,
-                  0);
-          }
-      }
-    }
-    if (kt + 1 < nk) store_tiles(buf ^ 1);
-    __syncthreads();
-  }
-
-  // epilogue: lane holds C[m = .. + l32][n = .. + 8*(q/4) + 4*lh + q%4]
-  const bool split = gridDim.z > 1;
-#pragma unroll
-  for (int i = 0; i < FM; ++i) {
-    const int64_t m = m0 + wm * WTM + i * 32 + l32;
-    if (m >= g.M) continue;
-#pragma unroll
-    for (int j = 0; j < FN; ++j) {
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const int64_t n = n0 + wn * WTN + j * 32 + 8 * (q >> 2) + 4 * lh + (q & 3);
-        if (n >= g.N) continue;
-        float v = acc[i][j][q];
-        if (split) {
-          g.W[((int64_t)blockIdx.z * g.N + n) * g.ldw + m] = g.alpha * v;
-        } else {
-          OutT* cp = (OutT*)g.C + m + n * g.ldc;
-          float r = (g.beta != 0.0f) ? __builtin_fmaf(g.alpha, v, g.beta * ld_out(cp)) : g.alpha * v;
-          *cp = (OutT)r;
-        }
-      }
-    }
-  }
-}
+#include "gemm_mfma.hpp"
 
 // C = beta*C + sum_z W[z] (slices added in order z = 0, 1, ...)
 template <typename OutT>
@@ -369,15 +100,25 @@ __global__ void __launch_bounds__(256) gemm_h16_kernel(int64_t M, int64_t N, int
     }
 }
 
+// K slab depth: 16 for fp32 operands; for 16-bit operands 64, except the
+// 256x256 tile (32: its 256 accumulators leave no room for deeper prefetch)
+constexpr int slab_k(int opk, int bm, int bn) {
+  return opk == OPK_F32 ? 16 : (bm * bn >= 256 * 256 ? 32 : 64);
+}
+
 template <int OPK, typename OutT, int BM, int BN, int WM, int WN>
 int launch_mfma(remifa_handle_t h, GemmArgs g, int split, bool vec) {
+  constexpr int BK = slab_k(OPK, BM, BN);
   g.tiles_m = (int)ceil_div(g.M, BM);
   g.tiles_n = (int)ceil_div(g.N, BN);
   dim3 grid((unsigned)(g.tiles_m * g.tiles_n), 1, (unsigned)split);
+  // accumulator preload from C: saves the epilogue read, costs registers the
+  // 256x256 tile does not have
+  constexpr bool PRE = BM * BN < 256 * 256;
   if (vec)
-    gemm_mfma_kernel<OPK, OutT, BM, BN, WM, WN, true><<<grid, 256, 0, h->stream>>>(g);
+    gemm_mfma_kernel<OPK, OutT, BM, BN, WM, WN, BK, true, PRE><<<grid, 256, 0, h->stream>>>(g);
   else
-    gemm_mfma_kernel<OPK, OutT, BM, BN, WM, WN, false><<<grid, 256, 0, h->stream>>>(g);
+    gemm_mfma_kernel<OPK, OutT, BM, BN, WM, WN, BK, false, PRE><<<grid, 256, 0, h->stream>>>(g);
   RF_HIP(hipGetLastError());
   return 0;
 }
@@ -462,9 +203,10 @@ int gemm_launch(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t m, in
   g.C = c;
   const size_t es = (opk == OPK_F32) ? 4 : 2;
   const int chunk = 16 / (int)es;
+  // whole 16-byte chunks: aligned operands, M and K multiples of the chunk
   const bool vec = ((uintptr_t)a % 16 == 0) && ((uintptr_t)b % 16 == 0) && (lda % chunk == 0) &&
-                   (ldb % chunk == 0);
-  const int bk = (opk == OPK_F32) ? 16 : 32;
+                   (ldb % chunk == 0) && (m % chunk == 0) && (g.K % chunk == 0);
+  const int bk = 64;  // split granularity: a multiple of every slab depth
 
   // split-K when the output tiles cannot fill the chip and K is long
   int split = h->split_k;
@@ -495,7 +237,10 @@ int gemm_launch(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t m, in
     g.ldw = m;
   }
 
-  const int tile = choose_tile(opk, m, n);
+  int tile = choose_tile(opk, m, n);
+  if (h->gemm_tile >= 0 && (opk != OPK_F32 || h->gemm_tile == T256x32 ||
+                            h->gemm_tile == T32x256 || h->gemm_tile == T128x128))
+    tile = h->gemm_tile;  // explicit choice (autotuner / experiments)
   timer_begin(h, KC_GEMM, flops, (opk << 8) | ((out_t == REMIFA_R_16F) << 4) | tile);
   int st;
   if (opk == OPK_F32) {

@@ -1,0 +1,319 @@
+// Matrix-core update GEMM kernel (included by gemm.hip inside its anonymous
+// namespace). C = alpha A B + beta C, column-major, fp32 accumulation.
+//
+// Structure per 256-thread workgroup (4 wave64s, one BM x BN tile of C):
+//  * The MFMA computes D^T = B^T A^T: its first operand is a B^T fragment
+//    (lane = column n of C), its second an A^T fragment (lane = row m of C),
+//    so a lane's accumulators cover 32 consecutive rows m across the wave and
+//    the C traffic coalesces along the column-major storage.
+//  * When alpha = +-1 the accumulators start at (beta/alpha) C, so the
+//    epilogue only stores.
+//  * K slabs of BK go HBM -> registers -> LDS with double buffering and one
+//    barrier per slab; the loads of slab k+1 are in flight while the matrix
+//    cores work on slab k. VEC kernels (16-byte aligned operands, M and K
+//    multiples of the 16-byte chunk) load whole chunks: a chunk is either in
+//    range or replaced by zeros through an address clamp and a select, so
+//    the loop body has no branches and the prefetch stays asynchronous.
+//    Other shapes use the element-wise VEC = false instantiation.
+//  * A slabs are stored m-contiguous ([k][m]) and B slabs k-contiguous
+//    ([n][k]), both with 16-byte stores; a B fragment is one 16-byte LDS
+//    read, an A fragment is gathered element-wise (the row padding keeps the
+//    two half-waves on different banks).
+//  * Workgroup ids are remapped so that every XCD (private L2) processes a
+//    contiguous range of tiles.
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+enum { OPK_F16 = 0, OPK_BF16 = 1, OPK_F32 = 2 };
+
+struct GemmArgs {
+  int64_t M, N, K;
+  int64_t lda, ldb, ldc;
+  float alpha, beta;
+  const void* A;
+  const void* B;
+  void* C;
+  float* W;        // split-K partials [split][N][ldw]
+  int64_t ldw;
+  int64_t kchunk;  // K range per split (multiple of BK)
+  int tiles_m, tiles_n;
+};
+
+template <int OPK>
+struct OpTraits;
+template <>
+struct OpTraits<OPK_F16> {
+  using Store = half_t;  // storage type in HBM
+  using Lds = uint16_t;  // element in LDS
+  static constexpr int CH = 8;  // elements per 16-byte chunk
+  using Chunk = u16x8;
+  static __device__ __forceinline__ Chunk cvt(Chunk c) { return c; }
+};
+template <>
+struct OpTraits<OPK_BF16> {
+  using Store = half_t;
+  using Lds = uint16_t;
+  static constexpr int CH = 8;
+  using Chunk = u16x8;
+  // fp16 storage -> bf16 operand, round to nearest even
+  static __device__ __forceinline__ Chunk cvt(Chunk c) {
+    Chunk r;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      float f = (float)__builtin_bit_cast(half_t, (uint16_t)c[v]);
+      r[v] = f32_to_bf16_bits(f);
+    }
+    return r;
+  }
+};
+template <>
+struct OpTraits<OPK_F32> {
+  using Store = float;
+  using Lds = float;
+  static constexpr int CH = 4;
+  using Chunk = f32x4;
+  static __device__ __forceinline__ Chunk cvt(Chunk c) { return c; }
+};
+
+// XCD-aware tile index: workgroup w runs on XCD w % 8; give every XCD a
+// contiguous range of tiles (a bijection for any tile count).
+__device__ __forceinline__ int xcd_tile(int w, int nt) {
+  int q = nt >> 3, r = nt & 7;
+  int x = w & 7, s = w >> 3;
+  return x * q + (x < r ? x : r) + s;
+}
+
+template <int OPK, typename OutT, int BM, int BN, int WM, int WN, int BK, bool VEC, bool PRELOAD>
+__global__ void __launch_bounds__(256) gemm_mfma_kernel(GemmArgs g) {
+  using TR = OpTraits<OPK>;
+  using S = typename TR::Store;
+  using L = typename TR::Lds;
+  using Chunk = typename TR::Chunk;
+  constexpr int CH = TR::CH;
+  constexpr int WTM = BM / WM, WTN = BN / WN;
+  constexpr int FM = WTM / 32, FN = WTN / 32;
+  constexpr int PADA = (OPK == OPK_F32) ? 32 : 8;
+  constexpr int PADB = (OPK == OPK_F32) ? 4 : 8;
+  constexpr int SA = BM + PADA;  // As[k][m]
+  constexpr int SB = BK + PADB;  // Bs[n][k]
+  constexpr int TA = BK * BM / CH, TB = BN * BK / CH;  // 16-byte chunks per slab
+  constexpr int NA = (TA + 255) / 256;
+  constexpr int NB = (TB + 255) / 256;
+  static_assert(TA % 64 == 0 && TB % 64 == 0, "slab chunks must fill whole waves");
+  static_assert(FM >= 1 && FN >= 1, "wave tile below 32x32");
+
+  __shared__ __attribute__((aligned(16))) L As[2][BK * SA];
+  __shared__ __attribute__((aligned(16))) L Bs[2][BN * SB];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  const int l32 = lane & 31, lh = lane >> 5;
+
+  const int nt = g.tiles_m * g.tiles_n;
+  const int tile = xcd_tile(blockIdx.x, nt);
+  const int tm = tile % g.tiles_m, tn = tile / g.tiles_m;
+  const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+  const int64_t kbeg = (int64_t)blockIdx.z * g.kchunk;
+  const int64_t kend = kbeg + g.kchunk < g.K ? kbeg + g.kchunk : g.K;
+  const int64_t klen = kend > kbeg ? kend - kbeg : 0;
+  const int nk = (int)((klen + BK - 1) / BK);
+  const bool split = gridDim.z > 1;
+
+  const S* __restrict__ A = (const S*)g.A;
+  const S* __restrict__ B = (const S*)g.B;
+
+  Chunk ra[NA], rb[NB];
+
+  auto load_slab = [&](int kt) {
+    const int64_t kk0 = kbeg + (int64_t)kt * BK;
+#pragma unroll
+    for (int p = 0; p < NA; ++p) {
+      int c = tid + 256 * p;
+      if (TA % 256 && c >= TA) break;
+      int mq = c % (BM / CH), k = c / (BM / CH);
+      int64_t gm = m0 + mq * CH, gk = kk0 + k;
+      if constexpr (VEC) {
+        bool ok = gm < g.M && gk < kend;
+        Chunk v = *(const Chunk*)(A + (ok ? gm + gk * g.lda : 0));
+        ra[p] = ok ? v : (Chunk)0;
+      } else {
+        Chunk v;
+#pragma unroll
+        for (int e = 0; e < CH; ++e) {
+          bool ok = gk < kend && gm + e < g.M;
+          S x = A[ok ? gm + e + gk * g.lda : 0];
+          if constexpr (OPK == OPK_F32)
+            v[e] = ok ? x : 0.0f;
+          else
+            v[e] = ok ? __builtin_bit_cast(uint16_t, x) : (uint16_t)0;
+        }
+        ra[p] = v;
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < NB; ++p) {
+      int c = tid + 256 * p;
+      if (TB % 256 && c >= TB) break;
+      int kq = c % (BK / CH), n = c / (BK / CH);
+      int64_t gk = kk0 + kq * CH, gn = n0 + n;
+      if constexpr (VEC) {
+        bool ok = gn < g.N && gk < kend;
+        Chunk v = *(const Chunk*)(B + (ok ? gk + gn * g.ldb : 0));
+        rb[p] = ok ? v : (Chunk)0;
+      } else {
+        Chunk v;
+#pragma unroll
+        for (int e = 0; e < CH; ++e) {
+          bool ok = gn < g.N && gk + e < kend;
+          S x = B[ok ? gk + e + gn * g.ldb : 0];
+          if constexpr (OPK == OPK_F32)
+            v[e] = ok ? x : 0.0f;
+          else
+            v[e] = ok ? __builtin_bit_cast(uint16_t, x) : (uint16_t)0;
+        }
+        rb[p] = v;
+      }
+    }
+  };
+  auto store_tiles = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < NA; ++p) {
+      int c = tid + 256 * p;
+      if (TA % 256 && c >= TA) break;
+      int mq = c % (BM / CH), k = c / (BM / CH);
+      *(Chunk*)&As[buf][k * SA + mq * CH] = TR::cvt(ra[p]);
+    }
+#pragma unroll
+    for (int p = 0; p < NB; ++p) {
+      int c = tid + 256 * p;
+      if (TB % 256 && c >= TB) break;
+      int kq = c % (BK / CH), n = c / (BK / CH);
+      *(Chunk*)&Bs[buf][n * SB + kq * CH] = TR::cvt(rb[p]);
+    }
+  };
+
+  f16v acc[FM][FN];
+  // accumulator start: (beta/alpha) C when that is exact, else 0
+  const bool from_c = PRELOAD && !split && g.beta != 0.0f && (g.alpha == 1.0f || g.alpha == -1.0f);
+  const float cscale = g.beta * g.alpha;  // beta/alpha for alpha = +-1
+  const OutT* __restrict__ Cr = (const OutT*)g.C;
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+    const int64_t m = m0 + wm * WTM + i * 32 + l32;
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int64_t nb = n0 + wn * WTN + j * 32 + 4 * lh;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int64_t n = nb + 8 * (q >> 2) + (q & 3);
+        bool ok = from_c && m < g.M && n < g.N;
+        float x = (float)Cr[ok ? m + n * g.ldc : 0];
+        acc[i][j][q] = ok ? cscale * x : 0.0f;
+      }
+    }
+  }
+
+  auto compute = [&](int buf) {
+    const L* as = As[buf];
+    const L* bs = Bs[buf];
+    if constexpr (OPK == OPK_F32) {
+#pragma unroll
+      for (int ks = 0; ks < BK; ks += 2) {
+        float xb[FN], ya[FM];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) xb[j] = bs[(wn * WTN + j * 32 + l32) * SB + ks + lh];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) ya[i] = as[(ks + lh) * SA + wm * WTM + i * 32 + l32];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(xb[j], ya[i], acc[i][j], 0, 0, 0);
+      }
+    } else {
+#pragma unroll
+      for (int ks = 0; ks < BK; ks += 16) {
+        u16x8 xb[FN], ya[FM];
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          xb[j] = *(const u16x8*)&bs[(wn * WTN + j * 32 + l32) * SB + ks + 8 * lh];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+#pragma unroll
+          for (int v = 0; v < 8; ++v) ya[i][v] = as[(ks + 8 * lh + v) * SA + wm * WTM + i * 32 + l32];
+        }
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) {
+            if constexpr (OPK == OPK_F16)
+              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
+                  __builtin_bit_cast(h8, xb[j]), __builtin_bit_cast(h8, ya[i]), acc[i][j], 0, 0, 0);
+            else
+              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                  __builtin_bit_cast(bf8, xb[j]), __builtin_bit_cast(bf8, ya[i]), acc[i][j], 0, 0,
+                  0);
+          }
+      }
+    }
+  };
+
+  if (nk > 0) {
+    load_slab(0);
+    store_tiles(0);
+  }
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int buf = kt & 1;
+    const bool more = kt + 1 < nk;
+    if (more) load_slab(kt + 1);
+    compute(buf);
+    if (more) store_tiles(buf ^ 1);
+    __syncthreads();
+  }
+
+  // epilogue: lane holds C[m = .. + l32][n = .. + 8*(q/4) + 4*lh + q%4]
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+    const int64_t m = m0 + wm * WTM + i * 32 + l32;
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int64_t nb = n0 + wn * WTN + j * 32 + 4 * lh;
+      if (split) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          const int64_t n = nb + 8 * (q >> 2) + (q & 3);
+          if (m < g.M && n < g.N)
+            g.W[((int64_t)blockIdx.z * g.N + n) * g.ldw + m] = g.alpha * acc[i][j][q];
+        }
+      } else if (from_c || g.beta == 0.0f) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          const int64_t n = nb + 8 * (q >> 2) + (q & 3);
+          if (m < g.M && n < g.N) ((OutT*)g.C)[m + n * g.ldc] = (OutT)(g.alpha * acc[i][j][q]);
+        }
+      } else {
+        // general alpha / beta: read this fragment of C, then write it
+        float cv[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          const int64_t n = nb + 8 * (q >> 2) + (q & 3);
+          bool ok = m < g.M && n < g.N;
+          cv[q] = (float)Cr[ok ? m + n * g.ldc : 0];
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          const int64_t n = nb + 8 * (q >> 2) + (q & 3);
+          float r = __builtin_fmaf(g.alpha, acc[i][j][q], g.beta * cv[q]);
+          if (m < g.M && n < g.N) ((OutT*)g.C)[m + n * g.ldc] = (OutT)r;
+        }
+      }
+    }
+  }
+}

@@ -133,6 +133,13 @@ int remifa_set_tuning(remifa_handle_t h, int panel_width, int split_k) {
   return 0;
 }
 
+int remifa_set_gemm_tile(remifa_handle_t h, int tile) {
+  if (!h) return -1;
+  if (tile < -1 || tile > 5) return -2;
+  h->gemm_tile = tile;
+  return 0;
+}
+
 int remifa_timing_enable(remifa_handle_t h, int enable) {
   if (!h) return -1;
   h->timer.enabled = enable != 0;
