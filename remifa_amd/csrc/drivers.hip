@@ -2,24 +2,35 @@
 //
 // LL (src/lu_nopiv_ll.hxx:55-582 with src/lu_nopiv_inner_ll.hxx:37-372) and
 // RL (src/lu_nopiv.hxx:53-208 with src/lu_nopiv_inner_rl.hxx:34-198) keep
-// the reference's loop structure and rounding points: for every outer block
-// an update GEMM (LL: K = ofs, RL: the Schur complement with K = nb), an
-// inner loop over ib-wide panels (diagonal-tile snapshot, panel kernel,
-// ib-wide updates) and the fp16 copies feeding the TC32 updates.
+// the reference's outer loops and rounding points: for every outer block of
+// nb columns an update GEMM (LL: block column and block row with K = ofs;
+// RL: the Schur complement with K = nb after the block) and the inner
+// factorization of the block with its ib-granular semantics.
+//
+// The inner factorization runs over slices of pw <= 64 columns (pw a multiple
+// of ib, default 64): each slice is one fused kernel (subpanel.hip) that does
+// the reference's ib loop inside, and the updates between slices are matrix-
+// core GEMMs (LL: the slice's columns/rows minus everything left of it, K =
+// slice offset; RL: a rank-pw update after it). pw = ib reproduces the
+// reference's structure of one panel per ib columns.
 //
 // Differences from the reference, all deliberate:
-//  * no allocation inside the loop: one workspace arena per call (snapshot,
+//  * no allocation inside the loop: one workspace arena per call (counter,
 //    fp32 panel buffers, fp16 copy when none is passed, split-K partials);
 //  * the precision tuple is a runtime argument (no template explosion);
 //  * d_info is 0 or the 1-based column of the first zero / non-finite pivot
 //    (the reference writes the last panel width, src/lu_nopiv.cu:173);
-//  * the inner U update reads and writes from column iofs + cblkn (the
-//    reference reads B from iofs + ib, which only agrees when nb % ib == 0).
+//  * the fp16 copy of the factors (TC updates on fp32 storage) is written by
+//    the slice kernel as it produces the factors, not by separate converts.
 #include <algorithm>
 
 #include "common.hpp"
 
 namespace remifa {
+
+int subpanel_launch(remifa_handle_t h, int T, int I, bool rl, int ib, int64_t cm, int sw, void* l,
+                    int64_t ldl, void* u, int64_t ldu, half_t* cp, int64_t ldcp, int* counter);
+
 namespace {
 
 __global__ void info_kernel(int64_t n, const void* a, int64_t lda, int t, int* info) {
@@ -36,7 +47,11 @@ __global__ void info_kernel(int64_t n, const void* a, int64_t lda, int t, int* i
   if (threadIdx.x == 0) *info = first == 0x7fffffff ? 0 : first;
 }
 
-bool is_tc(int c) { return c == REMIFA_TC32 || c == REMIFA_BF16_TC32 || c == REMIFA_FP8_TC32; }
+__global__ void zero_int_kernel(int* p, int n) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 bool is_h16acc(int c) { return c == REMIFA_FP16 || c == REMIFA_TC16; }
 
 // storage type of the operands a compute type reads
@@ -58,15 +73,15 @@ struct Ctx {
   remifa_handle_t h;
   remifa_lu_config_t c;
   int64_t ib;
-  void* dsnap;  // ib x ib diagonal-tile snapshot
+  int pw;        // slice width of the inner factorization
+  int* counter;  // slice kernel's diagonal-tile counter (kept at zero)
 };
 
 int gemm(Ctx& x, int ctype, int out_t, int64_t m, int64_t n, int64_t k, const void* a,
          int64_t lda, const void* b, int64_t ldb, void* c, int64_t ldc) {
   if (m <= 0 || n <= 0 || k <= 0) return 0;
-  int in_t = operand_type(ctype);
-  if (ctype == REMIFA_FP8_TC32) ctype = REMIFA_TC32;  // fp8 handled by the fused path only
-  return gemm_launch(x.h, ctype, in_t, out_t, m, n, k, -1.0f, a, lda, b, ldb, 1.0f, c, ldc);
+  return gemm_launch(x.h, ctype, operand_type(ctype), out_t, m, n, k, -1.0f, a, lda, b, ldb, 1.0f,
+                     c, ldc);
 }
 
 // element pointer arithmetic on untyped buffers
@@ -74,52 +89,38 @@ inline void* at(void* p, int t, int64_t i, int64_t j, int64_t ld) {
   return (char*)p + (i + j * ld) * (int64_t)dtype_size(t);
 }
 
-// inner left-looking factorization of one outer block: m x n block column
+// Inner left-looking factorization of one outer block: m x n block column
 // `l` and n x m block row `u` (columns n.. beyond the diagonal block), both
-// in precision T (l == u when factoring in place).
-// cpy/ldcp: fp16 mirror of this block's factors. When T = fp32 and the inner
-// update is a TC type its operands are read from the mirror (the reference's
-// d_a_cpy, src/lu_nopiv_inner_ll.hxx:225-265); `mirror` asks for the new
-// factors to be written to it after every panel.
+// in precision T (l == u when factoring in place). cpy/ldcp: fp16 mirror of
+// this block's factors; when T = fp32 and the inner update is a TC type its
+// operands are read from the mirror (the reference's d_a_cpy,
+// src/lu_nopiv_inner_ll.hxx:225-265). `mirror`: write the factors to cpy.
 int inner_ll(Ctx& x, int T, int64_t m, int64_t n, void* l, int64_t ldl, void* u, int64_t ldu,
              void* cpy, int64_t ldcp, bool mirror) {
   const int I = x.c.I;
-  const int64_t ib = x.ib;
-  const int64_t inc = (n - 1) / ib + 1;
   const bool tc_from_copy = (T == REMIFA_R_32F) && I != REMIFA_FP32;
   if (tc_from_copy) mirror = true;
-  for (int64_t k = 0; k < inc; ++k) {
-    const int64_t iofs = k * ib, cblkm = m - iofs;
-    const int64_t cblkn = std::min(n - iofs, ib);
-    if (iofs > 0) {
-      const void* la = tc_from_copy ? at(cpy, REMIFA_R_16F, iofs, 0, ldcp) : at(l, T, iofs, 0, ldl);
+  for (int64_t so = 0; so < n; so += x.pw) {
+    const int sw = (int)std::min<int64_t>(x.pw, n - so);
+    const int64_t cm = m - so;
+    if (so > 0) {
+      const void* la = tc_from_copy ? at(cpy, REMIFA_R_16F, so, 0, ldcp) : at(l, T, so, 0, ldl);
       const int64_t lla = tc_from_copy ? ldcp : ldl;
-      const void* ub = tc_from_copy ? at(cpy, REMIFA_R_16F, 0, iofs, ldcp) : at(u, T, 0, iofs, ldu);
+      const void* ub = tc_from_copy ? at(cpy, REMIFA_R_16F, 0, so, ldcp) : at(u, T, 0, so, ldu);
       const int64_t lub = tc_from_copy ? ldcp : ldu;
-      // L update (:223-246)
-      RF_CHECK(gemm(x, I, T, cblkm, cblkn, iofs, la, lla, ub, lub, at(l, T, iofs, iofs, ldl), ldl));
-      // U update (:249-301)
-      if (cblkm > cblkn) {
-        const void* ub2 = tc_from_copy ? at(cpy, REMIFA_R_16F, 0, iofs + cblkn, ldcp)
-                                       : at(u, T, 0, iofs + cblkn, ldu);
-        RF_CHECK(gemm(x, I, T, cblkn, cblkm - cblkn, iofs, la, lla, ub2, lub,
-                      at(u, T, iofs, iofs + cblkn, ldu), ldu));
+      // columns of the slice (:223-246)
+      RF_CHECK(gemm(x, I, T, cm, sw, so, la, lla, ub, lub, at(l, T, so, so, ldl), ldl));
+      // rows of the slice, right of its diagonal tile (:249-301)
+      if (cm > sw) {
+        const void* ub2 = tc_from_copy ? at(cpy, REMIFA_R_16F, 0, so + sw, ldcp)
+                                       : at(u, T, 0, so + sw, ldu);
+        RF_CHECK(gemm(x, I, T, sw, cm - sw, so, la, lla, ub2, lub, at(u, T, so, so + sw, ldu), ldu));
       }
     }
-    // diagonal snapshot + panel (:313-360)
-    RF_CHECK(copy2d_launch(x.h, dtype_size(T), cblkn, cblkn, at(l, T, iofs, iofs, ldl), ldl, x.dsnap,
-                           ib));
-    RF_CHECK(panel_launch(x.h, T, (int)ib, cblkm, cblkn, x.dsnap, ib, at(l, T, iofs, iofs, ldl), ldl,
-                          at(u, T, iofs, iofs, ldu), ldu, nullptr));
-    // fp16 copies of the new factors (:362-368)
-    if (mirror) {
-      if (cblkm > cblkn)
-        RF_CHECK(convert_launch(x.h, T, REMIFA_R_16F, cblkn, cblkm - cblkn,
-                                at(u, T, iofs, iofs + cblkn, ldu), ldu,
-                                at(cpy, REMIFA_R_16F, iofs, iofs + cblkn, ldcp), ldcp));
-      RF_CHECK(convert_launch(x.h, T, REMIFA_R_16F, cblkm, cblkn, at(l, T, iofs, iofs, ldl), ldl,
-                              at(cpy, REMIFA_R_16F, iofs, iofs, ldcp), ldcp));
-    }
+    RF_CHECK(subpanel_launch(x.h, T, I, false, (int)x.ib, cm, sw, at(l, T, so, so, ldl), ldl,
+                             at(u, T, so, so, ldu), ldu,
+                             mirror ? (half_t*)at(cpy, REMIFA_R_16F, so, so, ldcp) : nullptr, ldcp,
+                             x.counter));
   }
   return 0;
 }
@@ -129,7 +130,7 @@ int validate(const remifa_lu_config_t& c, int algo) {
   auto ct_ok = [](int t) { return t >= REMIFA_FP32 && t <= REMIFA_FP8_TC32; };
   if (!prec_ok(c.W) || !ct_ok(c.I) || !ct_ok(c.O)) return 1;
   if (!prec_ok(c.OP) || !prec_ok(c.IP) || !prec_ok(c.OF) || !prec_ok(c.IF)) return 1;
-  if (c.ib < 1 || c.ib > 32) return 1;
+  if (c.ib != 8 && c.ib != 16 && c.ib != 32 && c.ib != 64) return 1;
   if (c.W == REMIFA_R_32F) {
     // src/lu_nopiv_inner_ll.hxx:74-100: fp32 storage needs fp32 panels/factors
     if (c.OP != REMIFA_R_32F || c.IP != REMIFA_R_32F || c.OF != REMIFA_R_32F ||
@@ -145,6 +146,7 @@ int validate(const remifa_lu_config_t& c, int algo) {
       bool buffered = c.IP == REMIFA_R_32F && c.IF == REMIFA_R_32F && c.OF == c.OP;
       if (!inplace && !buffered) return 1;
       if (is_h16acc(c.O) && c.OP != REMIFA_R_16F) return 1;
+      if (buffered && is_h16acc(c.I)) return 1;
     }
   }
   // bf16 / fp8 updates read operands rounded from the fp16 storage
@@ -159,7 +161,7 @@ int validate(const remifa_lu_config_t& c, int algo) {
 // Workspace needed by lu_nopiv (bytes).
 size_t lu_workspace_bytes(int algo, const remifa_lu_config_t& c, int64_t m, int64_t n, int64_t nb,
                           bool have_cpy) {
-  size_t b = 256 + 32 * 32 * 8;  // snapshot
+  size_t b = 256;  // slice counter
   bool need_cpy = c.W == REMIFA_R_32F && (c.I != REMIFA_FP32 || c.O != REMIFA_FP32);
   if (need_cpy && !have_cpy) b += ((size_t)m * n * 2 + 255) & ~(size_t)255;
   if (algo == REMIFA_LL && c.W == REMIFA_R_16F && c.IP == REMIFA_R_32F) {
@@ -183,15 +185,20 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
   const remifa_lu_config_t c = *cfg;
   const int W = c.W;
   if (m == 0) {
-    if (d_info) RF_HIP(hipMemsetAsync(d_info, 0, sizeof(int), h->stream));
+    if (d_info) zero_int_kernel<<<1, 256, 0, h->stream>>>(d_info, 1);
+    RF_HIP(hipStreamSynchronize(h->stream));
     return 0;
   }
   const bool need_cpy = W == REMIFA_R_32F && (c.I != REMIFA_FP32 || c.O != REMIFA_FP32);
   const size_t ws = lu_workspace_bytes(algo, c, m, n, nb, d_cpy != nullptr);
   RF_CHECK(ensure_workspace(h, ws));
   Arena ar{(char*)h->ws};
-  Ctx x{h, c, c.ib, nullptr};
-  x.dsnap = ar.take(32 * 32 * 8);
+  // slice width: a multiple of ib, at most 64 (handle knob, default 64)
+  int pw = h->panel_width > 0 ? h->panel_width : 64;
+  pw = std::max<int>((int)c.ib, std::min(64, pw / (int)c.ib * (int)c.ib));
+  Ctx x{h, c, c.ib, pw, nullptr};
+  x.counter = (int*)ar.take(256);
+  zero_int_kernel<<<1, 256, 0, h->stream>>>(x.counter, 64);
   void* cpy = d_cpy;
   int64_t ldc = ldcpy;
   if (need_cpy && !cpy) {
@@ -206,7 +213,6 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
     pu = ar.take((size_t)nb * m * 4);
   }
   h->ws_used = ar.off;
-  int st = 0;
   const int64_t nc = (n - 1) / nb + 1;
 
   auto run = [&]() -> int {
@@ -230,17 +236,10 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
               RF_CHECK(gemm(x, c.O, W, in, updm - in, ofs, la, lo, ua2, lo,
                             at(d_a, W, ofs, ofs + in, ldda), ldda));
           }
+          // inner factorization in place; with an fp16 copy (W = fp32 and a
+          // TC update) the slice kernels keep it current (:554-563)
           void* cblk = need_cpy ? at(cpy, REMIFA_R_16F, ofs, ofs, ldc) : nullptr;
-          RF_CHECK(inner_ll(x, W, updm, in, blk, ldda, blk, ldda, cblk, ldc, false));
-          // fp16 copy of the new factors for later TC outer updates (:554-563)
-          if (need_cpy && c.I == REMIFA_FP32) {
-            if (updm > in)
-              RF_CHECK(convert_launch(h, W, REMIFA_R_16F, in, updm - in,
-                                      at(d_a, W, ofs, ofs + in, ldda), ldda,
-                                      at(cpy, REMIFA_R_16F, ofs, ofs + in, ldc), ldc));
-            RF_CHECK(convert_launch(h, W, REMIFA_R_16F, updm, in, blk, ldda,
-                                    at(cpy, REMIFA_R_16F, ofs, ofs, ldc), ldc));
-          }
+          RF_CHECK(inner_ll(x, W, updm, in, blk, ldda, blk, ldda, cblk, ldc, need_cpy));
         } else {
           // fp16 storage, fp32 inner buffers: pl = block column (updm x in),
           // pu = block row incl. the diagonal block (in x updm)
@@ -250,16 +249,18 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
               RF_CHECK(gemm(x, c.O, W, in, updm - in, ofs, la, lo, ua2, lo,
                             at(d_a, W, ofs, ofs + in, ldda), ldda));
           }
+          // (the diagonal block is in both buffers, as in the reference's
+          // d_l_tmp / d_u_tmp: L and the diagonal tiles are read from pl,
+          // the U part right of each tile from pu)
           RF_CHECK(convert_launch(h, W, REMIFA_R_32F, updm, in, blk, ldda, pl, ldpl));
           RF_CHECK(convert_launch(h, W, REMIFA_R_32F, in, updm, blk, ldda, pu, ldpu));
           if (c.OP == REMIFA_R_32F && ofs > 0) {
             RF_CHECK(gemm(x, c.O, REMIFA_R_32F, updm, in, ofs, la, lo, ua, lo, pl, ldpl));
-            if (updm > in)
-              RF_CHECK(gemm(x, c.O, REMIFA_R_32F, in, updm, ofs, la, lo, ua, lo, pu, ldpu));
+            RF_CHECK(gemm(x, c.O, REMIFA_R_32F, in, updm, ofs, la, lo, ua, lo, pu, ldpu));
           }
-          // inner factorization in fp32; the TC operands are the fp16 factors
-          // already written to A (the reference passes d_a as its fp16 copy,
-          // src/lu_nopiv_ll.cxx:632)
+          // inner factorization in fp32; its fp16 mirror is A itself (the
+          // reference passes d_a as the fp16 copy, src/lu_nopiv_ll.cxx:632),
+          // so the rounded factors land in A as they are produced
           RF_CHECK(inner_ll(x, REMIFA_R_32F, updm, in, pl, ldpl, pu, ldpu, blk, ldda, true));
         }
       }
@@ -271,41 +272,29 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
         const int64_t ofs = kk * nb, ofst = (kk + 1) * nb;
         const int64_t in = std::min(n - ofs, nb), updm = m - ofs;
         void* p = at(d_a, W, ofs, ofs, ldda);
-        const int64_t inc = (in - 1) / x.ib + 1;
-        for (int64_t k = 0; k < inc; ++k) {
-          const int64_t iofs = k * x.ib, cblkm = updm - iofs;
-          const int64_t cblkn = std::min(in - iofs, x.ib);
-          const int64_t iofst = (k + 1) * x.ib;
-          const int64_t tblkm = updm - iofst, tblkn = in - iofst;
-          void* b = at(p, W, iofs, iofs, ldda);
-          RF_CHECK(copy2d_launch(h, dtype_size(W), cblkn, cblkn, b, ldda, x.dsnap, x.ib));
-          RF_CHECK(panel_launch(h, W, (int)x.ib, cblkm, cblkn, x.dsnap, x.ib, b, ldda, b, ldda,
-                                nullptr));
-          if (tc_in || tc_out) {
-            // fp16 copy of the new ib column and row (:131-137)
-            if (cblkm > cblkn)
-              RF_CHECK(convert_launch(h, W, REMIFA_R_16F, cblkn, cblkm - cblkn,
-                                      at(p, W, iofs, iofs + cblkn, ldda), ldda,
-                                      at(cpy, REMIFA_R_16F, ofs + iofs, ofs + iofs + cblkn, ldc),
-                                      ldc));
-            RF_CHECK(convert_launch(h, W, REMIFA_R_16F, cblkm, cblkn, b, ldda,
-                                    at(cpy, REMIFA_R_16F, ofs + iofs, ofs + iofs, ldc), ldc));
-          }
-          if (tblkn > 0) {
-            const void* lop = tc_in ? at(cpy, REMIFA_R_16F, ofs + iofst, ofs + iofs, ldc)
-                                    : at(p, W, iofst, iofs, ldda);
-            const void* uop = tc_in ? at(cpy, REMIFA_R_16F, ofs + iofs, ofs + iofst, ldc)
-                                    : at(p, W, iofs, iofst, ldda);
+        for (int64_t so = 0; so < in; so += x.pw) {
+          const int sw = (int)std::min<int64_t>(x.pw, in - so);
+          const int64_t cm = updm - so, tso = so + sw;
+          RF_CHECK(subpanel_launch(
+              h, W, c.I, true, (int)x.ib, cm, sw, at(p, W, so, so, ldda), ldda,
+              at(p, W, so, so, ldda), ldda,
+              (tc_in || tc_out) ? (half_t*)at(cpy, REMIFA_R_16F, ofs + so, ofs + so, ldc) : nullptr,
+              ldc, x.counter));
+          if (in - tso > 0) {
+            const void* lop = tc_in ? at(cpy, REMIFA_R_16F, ofs + tso, ofs + so, ldc)
+                                    : at(p, W, tso, so, ldda);
+            const void* uop = tc_in ? at(cpy, REMIFA_R_16F, ofs + so, ofs + tso, ldc)
+                                    : at(p, W, so, tso, ldda);
             const int64_t lop_ld = tc_in ? ldc : ldda;
-            // rank-ib update of the outer block column (:139-163)
-            RF_CHECK(gemm(x, c.I, W, tblkm, tblkn, x.ib, lop, lop_ld, uop, lop_ld,
-                          at(p, W, iofst, iofst, ldda), ldda));
+            // rank-sw update of the outer block column (:139-163)
+            RF_CHECK(gemm(x, c.I, W, updm - tso, in - tso, sw, lop, lop_ld, uop, lop_ld,
+                          at(p, W, tso, tso, ldda), ldda));
             // and of the outer block row beyond the outer block (:167-193)
-            if (tblkm > tblkn) {
-              const void* uop2 = tc_in ? at(cpy, REMIFA_R_16F, ofs + iofs, ofst, ldc)
-                                       : at(p, W, iofs, nb, ldda);
-              RF_CHECK(gemm(x, c.I, W, tblkn, tblkm - tblkn, x.ib, lop, lop_ld, uop2, lop_ld,
-                            at(p, W, iofst, nb, ldda), ldda));
+            if (updm > in) {
+              const void* uop2 = tc_in ? at(cpy, REMIFA_R_16F, ofs + so, ofs + in, ldc)
+                                       : at(p, W, so, in, ldda);
+              RF_CHECK(gemm(x, c.I, W, in - tso, updm - in, sw, lop, lop_ld, uop2, lop_ld,
+                            at(p, W, tso, in, ldda), ldda));
             }
           }
         }
@@ -322,7 +311,7 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
     }
     return 0;
   };
-  st = run();
+  int st = run();
   h->ws_used = 0;
   if (st) return st;
   if (d_info) {
