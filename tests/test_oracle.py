@@ -1,0 +1,505 @@
+"""CPU tests that pin the oracle (the CPU restatement in oracle/) to facts that
+do not come from the oracle itself: the C++ standard library's generator,
+numpy/torch casts, exact rational arithmetic, closed forms and invariants.
+
+The reference holds no stored outputs (its tests print, they do not assert),
+so these are the checks that make a plausible mistake in the restatement (a
+dropped term, a wrong index or sign, a division where the reference
+multiplies by a reciprocal, unfused where it fuses) fail."""
+import json
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle as orc
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _built():
+    orc.build()
+    orc.set_threads(4)
+
+
+# ---------------------------------------------------------------- exact rounding
+def round_exact(q, p: int, emin: int, emax: int) -> float:
+    """Round a rational to the binary format with p significant bits, minimum
+    normal exponent emin and maximum exponent emax, to nearest even; overflow
+    to +-inf."""
+    q = Fraction(q)
+    if q == 0:
+        return 0.0
+    s = -1 if q < 0 else 1
+    a = abs(q)
+    e = a.numerator.bit_length() - a.denominator.bit_length()
+    if Fraction(2) ** e > a:
+        e -= 1
+    quantum = Fraction(2) ** (max(e, emin) - (p - 1))
+    n, rem = divmod(a, quantum)
+    n = int(n)
+    if rem * 2 > quantum or (rem * 2 == quantum and n % 2 == 1):
+        n += 1
+    r = n * quantum
+    if r >= Fraction(2) ** (emax + 1):
+        return s * float("inf")
+    return s * float(r)
+
+
+def rf32(q):
+    return round_exact(q, 24, -126, 127)
+
+
+def rf16(q):
+    return round_exact(q, 11, -14, 15)
+
+
+# ---------------------------------------------------------------- generator
+def _hex_to_f32(hs):
+    return np.array([int(h, 16) for h in hs], np.uint32).view(np.float32)
+
+
+def test_minstd_matches_libstdcxx():
+    """orc_minstd_stream reproduces std::default_random_engine +
+    uniform_real_distribution<float>(-1, 1) bit for bit (tests/genmat.hxx:72-73)."""
+    gold = json.load(open(os.path.join(GOLDEN, "minstd.json")))
+    for seed, hs in gold.items():
+        want = _hex_to_f32(hs)
+        got = orc.minstd_stream(1 if int(seed) < 0 else int(seed), len(want))
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), seed
+
+
+def test_gen_diagdom_layout():
+    """Draws fill A row by row (i outer, j inner) and a_ii = m
+    (tests/genmat.hxx:86-97)."""
+    m, n = 7, 5
+    a = orc.gen_diagdom(m, n)
+    s = orc.minstd_stream(1, m * n).reshape(m, n)
+    for i in range(m):
+        for j in range(n):
+            want = float(m) if i == j else s[i, j]
+            assert a[i, j] == want
+
+
+def test_gen_diagdom_alpha_diagonal():
+    n, seed, alpha = 64, 2024, 1.05
+    a = orc.gen_diagdom_alpha(n, seed, alpha)
+    s = orc.minstd_stream(seed, n * n).reshape(n, n).astype(np.float64)
+    off = ~np.eye(n, dtype=bool)
+    assert np.array_equal(a[off], s.astype(np.float32)[off])
+    rows = np.where(off, np.abs(s), 0.0).sum(axis=1)  # exact: dyadic values
+    want = np.float32(alpha * rows)
+    assert np.array_equal(np.diag(a), want)
+    # row diagonal dominance by the factor alpha
+    assert np.all(np.abs(np.diag(a)) >= alpha * rows * (1 - 2 ** -23))
+
+
+def test_minstd_jump_ahead_consistent():
+    """the per-row jump-ahead the parallel generator uses equals sequential draws"""
+    n = 33
+    a = orc.gen_diagdom_alpha(n, 99, 2.0)
+    s = orc.minstd_stream(99, n * n).reshape(n, n)
+    off = ~np.eye(n, dtype=bool)
+    assert np.array_equal(a[off], s[off])
+
+
+def test_gen_uniform_row_major_draws():
+    """right-hand sides are drawn like gen_diagdom: row by row"""
+    m, n = 6, 3
+    b = orc.gen_uniform(m, n, 5)
+    s = orc.minstd_stream(5, m * n)
+    assert np.array_equal(b.flatten(order="C"), s)
+
+
+# ---------------------------------------------------------------- conversions
+def test_f16_widening_exhaustive():
+    h = np.arange(65536, dtype=np.uint32).astype(np.uint16)
+    got = orc.f16_bits_to_f32(h.reshape(256, 256))
+    want = h.view(np.float16).astype(np.float32).reshape(256, 256)
+    nan = np.isnan(want)
+    assert np.array_equal(np.isnan(got), nan)
+    assert np.array_equal(got[~nan].view(np.uint32), want[~nan].view(np.uint32))
+
+
+def test_narrowing_truth_table():
+    g = json.load(open(os.path.join(GOLDEN, "convert.json")))
+    x = np.array(g["f32_bits"], np.uint32).view(np.float32)[None, :]
+    assert np.array_equal(orc.convert(x, orc.F32, orc.F16)[0], np.array(g["f16_bits"], np.uint16))
+    assert np.array_equal(orc.convert(x, orc.F32, orc.BF16)[0],
+                          np.array(g["bf16_bits"], np.uint16))
+    xe = np.array(g["e4m3_f32_bits"], np.uint32).view(np.float32)[None, :]
+    assert np.array_equal(orc.convert(xe, orc.F32, orc.E4M3)[0], np.array(g["e4m3_bits"], np.uint8))
+
+
+def test_f32_to_f16_random_patterns_vs_numpy():
+    rng = np.random.default_rng(0)
+    bits = rng.integers(0, 2 ** 32, size=(512, 512), dtype=np.uint64).astype(np.uint32)
+    x = bits.view(np.float32)
+    with np.errstate(over="ignore", invalid="ignore"):
+        want = x.astype(np.float16)
+    got = orc.convert(x, orc.F32, orc.F16).view(np.float16)
+    nan = np.isnan(want)
+    assert np.array_equal(np.isnan(got), nan)
+    assert np.array_equal(got[~nan].view(np.uint16), want[~nan].view(np.uint16))
+
+
+def test_f64_to_f16_direct_rounding():
+    """double -> half rounds once (numpy's direct cast), not via float."""
+    rng = np.random.default_rng(1)
+    x = (rng.standard_normal((64, 64)) * 10.0 ** rng.integers(-9, 5, (64, 64)))
+    # a value whose double->float->half double rounding differs
+    x[0, 0] = 1.0 + 2.0 ** -11 + 2.0 ** -40
+    want = x.astype(np.float16)
+    got = orc.convert(x, orc.F64, orc.F16).view(np.float16)
+    assert np.array_equal(got.view(np.uint16), want.view(np.uint16))
+    assert got[0, 0] == np.float16(1.0 + 2.0 ** -10)
+
+
+def test_e4m3_decode_all_codes():
+    torch = pytest.importorskip("torch")
+    codes = np.arange(256, dtype=np.uint8)
+    want = torch.from_numpy(codes.copy()).view(torch.float8_e4m3fn).float().numpy()
+    got = orc.convert(codes[None, :], orc.E4M3, orc.F32)[0]
+    nan = np.isnan(want)
+    assert np.array_equal(np.isnan(got), nan)
+    assert np.array_equal(got[~nan], want[~nan])
+
+
+def test_e4m3_saturates():
+    x = np.array([[1000.0, -1000.0, 449.0, np.inf]], np.float32)
+    got = orc.convert(orc.convert(x, orc.F32, orc.E4M3), orc.E4M3, orc.F32)[0]
+    assert list(got[:3]) == [448.0, -448.0, 448.0]
+
+
+# ---------------------------------------------------------------- panel
+def test_panel_rcp_then_multiply_fused_update():
+    """2x2 tile: l21 = a21 * rn(1/a11) (not a21/a11) and u22 = fma(-l21, a12, a22)
+    rounded once (src/lu_nopiv.cu:138-167)."""
+    a11, a12, a21 = np.float32(3.0), np.float32(1.0 + 2.0 ** -20), np.float32(5.0)
+    a22 = np.float32(1.0)
+    d = np.array([[a11, a12], [a21, a22]], np.float32)
+    l, _, stat = orc.lu_nopiv_panel(orc.F32, 8, 2, 2, d, d.copy(), np.zeros((2, 2), np.float32))
+    r = rf32(Fraction(1) / Fraction(float(a11)))
+    l21 = rf32(Fraction(float(a21)) * Fraction(r))
+    assert l21 != rf32(Fraction(5, 3))  # the two orders differ here
+    assert l[1, 0] == np.float32(l21)
+    u22 = rf32(Fraction(float(a22)) - Fraction(l21) * Fraction(float(a12)))
+    unfused = np.float32(np.float32(a22) - np.float32(np.float32(l21) * a12))
+    assert u22 != float(unfused)
+    assert l[1, 1] == np.float32(u22)
+    assert stat == 2
+
+
+def test_panel_f16_rounds_each_operation():
+    a11, a21 = 3.0, 5.0
+    d = np.array([[a11, 1.0], [a21, 1.0]])
+    dh = orc.f32_to_f16_bits(d.astype(np.float32))
+    l, _, _ = orc.lu_nopiv_panel(orc.F16, 8, 2, 2, dh, dh.copy(), np.zeros((2, 2), np.uint16))
+    r = rf16(Fraction(1, 3))
+    want = rf16(Fraction(5) * Fraction(r))
+    assert orc.f16_bits_to_f32(l)[1, 0] == np.float32(want)
+
+
+def _exact_factors(n, seed, ib_rows=0):
+    """unit lower L0 and upper U0 with dyadic entries (|l| <= 1/2 on a band,
+    u_kk a power of two) so every product and partial sum of the elimination
+    is exact in fp16 and fp32."""
+    rng = np.random.default_rng(seed)
+    L = np.eye(n)
+    U = np.zeros((n, n))
+    for i in range(n):
+        for j in range(max(0, i - 2), i):
+            L[i, j] = rng.choice([-0.5, 0.0, 0.25, 0.5])
+        U[i, i] = rng.choice([2.0, 4.0, -2.0])
+        for j in range(i + 1, min(n, i + 3)):
+            U[i, j] = rng.choice([-1.0, 0.0, 0.5, 1.0])
+    return L, U
+
+
+@pytest.mark.parametrize("prec", [orc.F32, orc.F16])
+def test_panel_exact_arithmetic(prec):
+    ib, m = 8, 24
+    L, U = _exact_factors(m, 3)
+    A = L @ U
+    d = A[:ib, :ib]
+    if prec == orc.F16:
+        cv = lambda x: orc.f32_to_f16_bits(x.astype(np.float32))
+        back = orc.f16_bits_to_f32
+    else:
+        cv = lambda x: x.astype(np.float32)
+        back = lambda x: x
+    l_in = A[:, :ib]
+    u_in = A[:ib, :]
+    l, u, _ = orc.lu_nopiv_panel(prec, ib, m, ib, cv(d), cv(l_in), cv(u_in))
+    lu_tile = np.tril(L[:ib, :ib], -1) + np.triu(U[:ib, :ib])
+    assert np.array_equal(back(l)[:ib], lu_tile.astype(np.float32))
+    assert np.array_equal(back(l)[ib:], L[ib:, :ib].astype(np.float32))
+    # U12 = L11^-1 A12 for the columns right of the tile
+    assert np.array_equal(back(u)[:, ib:], U[:ib, ib:].astype(np.float32))
+
+
+def test_panel_matches_f64_doolittle():
+    rng = np.random.default_rng(4)
+    ib, m = 16, 200
+    A = rng.uniform(-1, 1, (m, m)).astype(np.float32)
+    A[np.arange(ib), np.arange(ib)] += 8.0
+    l, u, _ = orc.lu_nopiv_panel(orc.F32, ib, m, ib, A[:ib, :ib], A[:, :ib], A[:ib, :])
+    # textbook Doolittle of the first ib columns in double
+    T = A[:ib, :ib].astype(np.float64)
+    Lt = np.eye(ib)
+    Ut = np.zeros((ib, ib))
+    for k in range(ib):
+        Ut[k, k:] = T[k, k:] - Lt[k, :k] @ Ut[:k, k:]
+        Lt[k + 1:, k] = (T[k + 1:, k] - Lt[k + 1:, :k] @ Ut[:k, k]) / Ut[k, k]
+    L21 = A[ib:, :ib].astype(np.float64) @ np.linalg.inv(Ut)
+    U12 = np.linalg.solve(Lt, A[:ib, ib:].astype(np.float64))
+    tol = 64 * 2.0 ** -24
+    assert np.allclose(l[:ib], np.tril(Lt, -1) + Ut, rtol=0, atol=tol * 8)
+    assert np.allclose(l[ib:], L21, rtol=0, atol=tol * 8)
+    assert np.allclose(u[:, ib:], U12, rtol=0, atol=tol * 8)
+
+
+# ---------------------------------------------------------------- GEMM
+def test_gemm_exact_small_integers():
+    rng = np.random.default_rng(6)
+    a = rng.integers(-4, 5, (37, 29)).astype(np.float32)
+    b = rng.integers(-4, 5, (29, 23)).astype(np.float32)
+    c = rng.integers(-50, 50, (37, 23)).astype(np.float32)
+    want = c - a.astype(np.float64) @ b
+    for ct in (orc.FP32, orc.TC32, orc.FP16, orc.TC16, orc.BF16TC32):
+        assert np.array_equal(orc.gemm_sub(ct, a, b, c), want.astype(np.float32)), ct
+
+
+@pytest.mark.parametrize("ct,opp", [(orc.FP32, orc.F32), (orc.TC32, orc.F16),
+                                    (orc.BF16TC32, orc.BF16)])
+def test_gemm_error_bound(ct, opp):
+    """|C - C^| <= gamma_{k+1} (|A~||B~| + |C|) with the operands rounded to the
+    compute type's input precision (A~, B~) and fp32 accumulation."""
+    rng = np.random.default_rng(7)
+    m, n, k = 64, 48, 300
+    a = rng.uniform(-1, 1, (m, k)).astype(np.float32)
+    b = rng.uniform(-1, 1, (k, n)).astype(np.float32)
+    c = rng.uniform(-1, 1, (m, n)).astype(np.float32)
+    at = orc.round_to(opp, a).astype(np.float64)
+    bt = orc.round_to(opp, b).astype(np.float64)
+    exact = c - at @ bt
+    got = orc.gemm_sub(ct, a, b, c)
+    u = 2.0 ** -24
+    g = (k + 1) * u / (1 - (k + 1) * u)
+    bound = g * (np.abs(at) @ np.abs(bt) + np.abs(c))
+    assert np.all(np.abs(got - exact) <= bound)
+    # and the operand rounding is really applied: unrounded operands miss
+    if opp != orc.F32:
+        near = np.max(np.abs(got - exact))
+        assert np.max(np.abs(got - (c - a.astype(np.float64) @ b))) > 100 * near
+
+
+def test_gemm_fp16_accumulation_brute_force():
+    """FP16 compute: products summed from zero in k order, each step one fp16
+    rounding, then C - acc rounded to C's precision."""
+    rng = np.random.default_rng(8)
+    m, n, k = 3, 2, 40
+    a = orc.round_to(orc.F16, rng.uniform(-1, 1, (m, k)))
+    b = orc.round_to(orc.F16, rng.uniform(-1, 1, (k, n)))
+    c = orc.round_to(orc.F16, rng.uniform(-1, 1, (m, n)))
+    got = orc.gemm_sub(orc.FP16, a, b, c, orc.F16)
+    for i in range(m):
+        for j in range(n):
+            acc = 0.0
+            for q in range(k):
+                acc = rf16(Fraction(float(a[i, q])) * Fraction(float(b[q, j])) + Fraction(acc))
+            want = rf16(Fraction(float(c[i, j])) - Fraction(acc))
+            assert got[i, j] == np.float32(want)
+
+
+def test_gemm_fp8_per_block_scale_bound():
+    rng = np.random.default_rng(9)
+    m, n, k = 32, 32, 128
+    a = rng.uniform(-1, 1, (m, k)).astype(np.float32)
+    b = rng.uniform(-1, 1, (k, n)).astype(np.float32)
+    c = np.zeros((m, n), np.float32)
+    got = orc.gemm_sub(orc.FP8TC32, a, b, c)
+    exact = -(a.astype(np.float64) @ b)
+    # e4m3 keeps 4 significant bits: relative operand error <= 2^-4
+    bound = 2 * 2.0 ** -4 * (np.abs(a) @ np.abs(b)) + 1e-5
+    assert np.all(np.abs(got - exact) <= bound)
+
+
+# ---------------------------------------------------------------- drivers
+CFGS = [
+    ("f32", dict(W=orc.F32, I=orc.FP32, O=orc.FP32)),
+    ("f32_tc32", dict(W=orc.F32, I=orc.TC32, O=orc.TC32)),
+    ("f16_tc32_fp32buf", dict(W=orc.F16, I=orc.TC32, O=orc.TC32, OP=orc.F32)),
+    ("f16_tc32", dict(W=orc.F16, I=orc.TC32, O=orc.TC32)),
+    ("f16_fp16", dict(W=orc.F16, I=orc.FP16, O=orc.FP16)),
+]
+
+
+def _prep(kw, a):
+    return orc.convert(a.astype(np.float32), orc.F32, orc.F16) if kw["W"] == orc.F16 else a
+
+
+def _unprep(kw, a):
+    return orc.f16_bits_to_f32(a) if kw["W"] == orc.F16 else a
+
+
+@pytest.mark.parametrize("name,kw", CFGS)
+@pytest.mark.parametrize("n,nb,ib", [(24, 16, 8), (100, 32, 8), (64, 64, 16)])
+def test_drivers_exact_arithmetic(name, kw, n, nb, ib):
+    """A = L0 U0 with every intermediate exact: LL and RL return L0 \\ U0."""
+    L, U = _exact_factors(n, n)
+    A = (L @ U).astype(np.float32)
+    want = (np.tril(L, -1) + U).astype(np.float32)
+    for algo in ("ll", "rl"):
+        if algo == "ll" or kw.get("OP", kw["W"]) == kw["W"]:
+            c = orc.cfg(ib=ib, **kw)
+            got = _unprep(kw, orc.lu_nopiv(algo, c, _prep(kw, A), nb))
+            assert np.array_equal(got, want), (algo, name)
+
+
+def _lu_parts(lu):
+    return np.tril(lu, -1) + np.eye(lu.shape[0]), np.triu(lu)
+
+
+def test_ll_f32_componentwise_bound_and_growth():
+    n, nb = 256, 64
+    a = orc.gen_diagdom_alpha(n, 3, 1.5)
+    lu = orc.lu_nopiv("ll", orc.cfg(ib=8), a, nb)
+    L, U = _lu_parts(lu.astype(np.float64))
+    u = 2.0 ** -24
+    g = n * u / (1 - n * u)
+    assert np.all(np.abs(a - L @ U) <= g * (np.abs(L) @ np.abs(U)))
+    # growth factor of a diagonally dominant matrix is at most 2
+    assert np.max(np.abs(U)) <= 2 * np.max(np.abs(a))
+
+
+def test_ll_rl_agree_with_f64_lu():
+    n, nb = 192, 64
+    a = orc.gen_diagdom_alpha(n, 12, 1.1)
+    ref = orc.lu_f64(a)
+    for algo in ("ll", "rl"):
+        lu = orc.lu_nopiv(algo, orc.cfg(ib=8), a, nb)
+        err = np.max(np.abs(lu - ref)) / np.max(np.abs(ref))
+        assert err < 64 * 2.0 ** -24, algo
+
+
+def test_f64_lu_textbook():
+    rng = np.random.default_rng(10)
+    a = rng.uniform(-1, 1, (9, 9)) + 9 * np.eye(9)
+    lu = orc.lu_f64(a)
+    L, U = _lu_parts(lu)
+    assert np.allclose(L @ U, a, atol=1e-14)
+
+
+def test_fp16_storage_error_scales_with_fp16_unit_roundoff():
+    n, nb = 256, 64
+    a = orc.gen_diagdom_alpha(n, 5, 1.05)
+    a16 = orc.convert(a, orc.F32, orc.F16)
+    ref = orc.lu_f64(orc.f16_bits_to_f32(a16))
+    c = orc.cfg(W=orc.F16, I=orc.TC32, O=orc.TC32, OP=orc.F32)
+    lu = orc.f16_bits_to_f32(orc.lu_nopiv("ll", c, a16, nb))
+    u16 = 2.0 ** -11
+    for part in (lambda x: np.tril(x, -1), np.triu):
+        err = np.max(np.abs(part(lu - ref))) / np.max(np.abs(part(ref)))
+        # stored in fp16: about one fp16 rounding, far above fp32's
+        assert u16 / 8 < err < 4 * u16
+    lu32 = orc.lu_nopiv("ll", orc.cfg(ib=8), orc.f16_bits_to_f32(a16), nb)
+    assert np.max(np.abs(lu32 - ref)) / np.max(np.abs(ref)) < 64 * 2.0 ** -24
+
+
+def test_drivers_bad_arguments():
+    a = np.eye(8, dtype=np.float32)
+    with pytest.raises(ValueError):
+        orc.lu_nopiv("ll", orc.cfg(ib=0), a, 4)
+
+
+# ---------------------------------------------------------------- errors
+def test_backward_error_closed_form_identity():
+    """A = I, b = x: nwerr = ||x^ - x||_inf / (||x^||_inf + ||x||_inf) (tests/errors.hxx:237)."""
+    n = 5
+    x = np.array([1.0, -2.0, 0.5, 3.0, -1.0])
+    e = np.array([1e-3, 0, -2e-3, 0, 0])
+    xh = x + e
+    got = orc.backward_error(np.eye(n, dtype=np.float32), x, xh)
+    want = np.max(np.abs(e)) / (np.max(np.abs(xh)) + np.max(np.abs(x)))
+    assert got == pytest.approx(want, rel=1e-14)
+    assert orc.backward_error(np.eye(n, dtype=np.float32), x, x) == 0.0
+
+
+def test_backward_error_scale_invariant():
+    rng = np.random.default_rng(11)
+    a = rng.uniform(-1, 1, (20, 20)).astype(np.float32) + 20 * np.eye(20, dtype=np.float32)
+    x = rng.uniform(-1, 1, 20)
+    b = a.astype(np.float64) @ np.ones(20)
+    e1 = orc.backward_error(a, b, x)
+    e2 = orc.backward_error(4 * a, 4 * b, x)
+    assert e1 == pytest.approx(e2, rel=1e-12)
+
+
+def test_compwise_error_closed_form_identity():
+    """A = LU = I: cwerr = max_i |e_i| / (2|x^_i|), Oettli-Prager form
+    max_i |e_i| / |x^_i| (tests/errors.hxx:147-166)."""
+    x = np.array([1.0, -2.0, 0.5])
+    e = np.array([1e-3, -1e-3, 1e-4])
+    xh = x + e
+    eye = np.eye(3, dtype=np.float32)
+    cw, cw_ax = orc.compwise_error(eye, x, xh, eye)
+    r = np.abs(e) / np.abs(xh)
+    assert cw == pytest.approx(np.max(r) / 2, rel=1e-12)
+    assert cw_ax == pytest.approx(np.max(r), rel=1e-12)
+
+
+def test_factor_error_zero_for_exact_factors():
+    L, U = _exact_factors(12, 1)
+    a = (L @ U).astype(np.float32)
+    lu = (np.tril(L, -1) + U).astype(np.float32)
+    assert orc.factor_error(a, lu) == 0.0
+    lu2 = lu.copy()
+    lu2[5, 3] += 0.25
+    assert orc.factor_error(a, lu2) > 0.0
+
+
+# ---------------------------------------------------------------- residual / IR
+def test_residual_f64_vs_exact():
+    rng = np.random.default_rng(12)
+    n = 24
+    a = rng.uniform(-1, 1, (n, n)).astype(np.float32)
+    x = rng.uniform(-1, 1, n)
+    b = rng.uniform(-1, 1, n)
+    r = orc.residual_f64(a, x, b)[:, 0]
+    for i in range(n):
+        exact = Fraction(float(b[i])) - sum(Fraction(float(a[i, j])) * Fraction(float(x[j]))
+                                            for j in range(n))
+        scale = abs(b[i]) + np.abs(a[i].astype(np.float64)) @ np.abs(x)
+        assert abs(r[i] - float(exact)) <= (n + 1) * 2.0 ** -53 * scale
+
+
+def test_getrs_f64_and_transposed():
+    rng = np.random.default_rng(13)
+    n = 30
+    a = rng.uniform(-1, 1, (n, n)) + n * np.eye(n)
+    lu = orc.lu_f64(a)
+    b = rng.uniform(-1, 1, (n, 3))
+    x = orc.getrs_f64(lu, b)
+    assert np.allclose(a @ x, b, atol=1e-13)
+    xt = orc.getrs_f64(lu, b, trans=True)
+    assert np.allclose(a.T @ xt, b, atol=1e-13)
+
+
+def test_ir_trace_fixture():
+    g = json.load(open(os.path.join(GOLDEN, "ir_trace.json")))
+    n, seed = g["n"], g["seed"]
+    a = orc.gen_diagdom_alpha(n, seed, g["alpha"])
+    a16 = orc.convert(a, orc.F32, orc.F16)
+    c = orc.cfg(W=orc.F16, I=orc.TC32, O=orc.TC32, OP=orc.F32, ib=g["ib"])
+    lu = orc.f16_bits_to_f32(orc.lu_nopiv("ll", c, a16, g["nb"]))
+    b = orc.gen_uniform(n, 1, seed + 1).astype(np.float64)
+    x, it, hist = orc.ir_solve(a, lu, b, tol=1e-14, maxit=30)
+    assert it == g["iters"]
+    assert np.allclose(hist, g["history"], rtol=1e-6, atol=0)
+    # converged: the restated error measure agrees
+    assert orc.backward_error(a, b, x) <= 1e-14
+    # refinement contracts by about the fp16 factorization's error each step
+    assert all(h1 < h0 * 1e-2 for h0, h1 in zip(hist[:-1], hist[1:]))
