@@ -15,8 +15,8 @@
 // reference's structure of one panel per ib columns.
 //
 // Differences from the reference, all deliberate:
-//  * no allocation inside the loop: one workspace arena per call (counter,
-//    fp32 panel buffers, fp16 copy when none is passed, split-K partials);
+//  * no allocation inside the loop: one workspace arena per call (slice
+//    table, fp32 panel buffers, fp16 copy when none is passed, split-K partials);
 //  * the precision tuple is a runtime argument (no template explosion);
 //  * d_info is 0 or the 1-based column of the first zero / non-finite pivot
 //    (the reference writes the last panel width, src/lu_nopiv.cu:173);
@@ -29,7 +29,8 @@
 namespace remifa {
 
 int subpanel_launch(remifa_handle_t h, int T, int I, bool rl, int ib, int64_t cm, int sw, void* l,
-                    int64_t ldl, void* u, int64_t ldu, half_t* cp, int64_t ldcp, int* counter);
+                    int64_t ldl, void* u, int64_t ldu, half_t* cp, int64_t ldcp, float* tab);
+size_t subpanel_table_bytes();
 
 namespace {
 
@@ -73,8 +74,8 @@ struct Ctx {
   remifa_handle_t h;
   remifa_lu_config_t c;
   int64_t ib;
-  int pw;        // slice width of the inner factorization
-  int* counter;  // slice kernel's diagonal-tile counter (kept at zero)
+  int pw;      // slice width of the inner factorization
+  float* tab;  // the slice's published diagonal tile
 };
 
 int gemm(Ctx& x, int ctype, int out_t, int64_t m, int64_t n, int64_t k, const void* a,
@@ -120,7 +121,7 @@ int inner_ll(Ctx& x, int T, int64_t m, int64_t n, void* l, int64_t ldl, void* u,
     RF_CHECK(subpanel_launch(x.h, T, I, false, (int)x.ib, cm, sw, at(l, T, so, so, ldl), ldl,
                              at(u, T, so, so, ldu), ldu,
                              mirror ? (half_t*)at(cpy, REMIFA_R_16F, so, so, ldcp) : nullptr, ldcp,
-                             x.counter));
+                             x.tab));
   }
   return 0;
 }
@@ -161,7 +162,7 @@ int validate(const remifa_lu_config_t& c, int algo) {
 // Workspace needed by lu_nopiv (bytes).
 size_t lu_workspace_bytes(int algo, const remifa_lu_config_t& c, int64_t m, int64_t n, int64_t nb,
                           bool have_cpy) {
-  size_t b = 256;  // slice counter
+  size_t b = (subpanel_table_bytes() + 255) & ~(size_t)255;  // slice table
   bool need_cpy = c.W == REMIFA_R_32F && (c.I != REMIFA_FP32 || c.O != REMIFA_FP32);
   if (need_cpy && !have_cpy) b += ((size_t)m * n * 2 + 255) & ~(size_t)255;
   if (algo == REMIFA_LL && c.W == REMIFA_R_16F && c.IP == REMIFA_R_32F) {
@@ -197,8 +198,7 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
   int pw = h->panel_width > 0 ? h->panel_width : 64;
   pw = std::max<int>((int)c.ib, std::min(64, pw / (int)c.ib * (int)c.ib));
   Ctx x{h, c, c.ib, pw, nullptr};
-  x.counter = (int*)ar.take(256);
-  zero_int_kernel<<<1, 256, 0, h->stream>>>(x.counter, 64);
+  x.tab = (float*)ar.take(subpanel_table_bytes());
   void* cpy = d_cpy;
   int64_t ldc = ldcpy;
   if (need_cpy && !cpy) {
@@ -279,7 +279,7 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
               h, W, c.I, true, (int)x.ib, cm, sw, at(p, W, so, so, ldda), ldda,
               at(p, W, so, so, ldda), ldda,
               (tc_in || tc_out) ? (half_t*)at(cpy, REMIFA_R_16F, ofs + so, ofs + so, ldc) : nullptr,
-              ldc, x.counter));
+              ldc, x.tab));
           if (in - tso > 0) {
             const void* lop = tc_in ? at(cpy, REMIFA_R_16F, ofs + tso, ofs + so, ldc)
                                     : at(p, W, tso, so, ldda);

@@ -1,15 +1,19 @@
 // Triangular solves with the stored no-pivot factors, the FP64 residual and
 // iterative refinement.
 //
-// TRSM (forward L, backward U and the transposed pair for A^T): one launch
-// per triangle. Workgroups take 64-row block rows in dependency order from an
-// atomic ticket, so every block a workgroup waits for belongs to a workgroup
-// that is already running (no deadlock whatever the residency). Block row i
-// accumulates B_i - sum_j T_ij X_j, waiting on a per-block flag (agent-scope
-// release/acquire) for each X_j it needs, then solves its diagonal block in
-// LDS and publishes X_i. Every wait is bounded: on timeout the kernel records
-// an error and exits, so the grid always drains. The factors are read once
-// (HBM-bound); the critical path is one diagonal-block solve per block row.
+// TRSM (forward L, backward U and the transposed pair for A^T): the 64 x 64
+// diagonal blocks are inverted first, all in parallel (diag_inv_kernel), so
+// the sequential chain of the substitution is one small matrix-vector product
+// per block row instead of 64 dependent steps. Then one launch per triangle:
+// workgroups take 64-row block rows in dependency order from an atomic
+// ticket, so every block a workgroup waits for belongs to a workgroup that is
+// already running (no deadlock whatever the residency). Blocks finish in
+// order, so one published-block counter (agent-scope release/acquire) tells
+// a workgroup how many solution blocks it may read; block row i accumulates
+// B_i - sum_j T_ij X_j over them, multiplies by its inverted diagonal block
+// and publishes X_i. Every wait is bounded: on timeout the kernel records an
+// error and exits, so the grid always drains. The factors are read once
+// (HBM-bound).
 //
 // Residual R = B - A X in FP64: 2-D grid over (256-row block, 1024-column
 // chunk), fp64 fma per element, per-chunk partials summed in chunk order by a
@@ -37,109 +41,190 @@ __device__ __forceinline__ float tel(const LT* lu, int64_t ld, int64_t r, int64_
   return TRANS ? ldf(lu + c + r * ld) : ldf(lu + r + c * ld);
 }
 
-template <typename LT, bool LOWER, bool UNIT, bool TRANS>
+// Inverses of the diagonal blocks: blockIdx.y = 0 for the unit lower L_bb,
+// 1 for the upper U_bb. inv[b][r][c] (64 x 64 row-major per block). Lane c
+// computes column c of the inverse by substitution with the block in LDS;
+// rows/columns past n are the identity.
+template <typename LT>
+__global__ void __launch_bounds__(64) diag_inv_kernel(int64_t n, const LT* __restrict__ lu,
+                                                      int64_t ld, float* __restrict__ inv_l,
+                                                      float* __restrict__ inv_u) {
+  __shared__ float D[NBS][NBS + 4];
+  const int t = threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.x * NBS;
+  const bool upper = blockIdx.y == 1;
+  for (int e = t; e < NBS * NBS; e += 64) {
+    const int r = e % NBS, c = e / NBS;
+    const int64_t R = r0 + r, C = r0 + c;
+    D[r][c] = (R < n && C < n) ? ldf(lu + R + C * ld) : (r == c ? 1.0f : 0.0f);
+  }
+  __syncthreads();
+  float x[NBS];
+  if (!upper) {
+#pragma unroll
+    for (int r = 0; r < NBS; ++r) {
+      float s = (r == t) ? 1.0f : 0.0f;
+#pragma unroll
+      for (int k = 0; k < r; ++k) s = __builtin_fmaf(-D[r][k], x[k], s);
+      x[r] = s;
+    }
+  } else {
+#pragma unroll
+    for (int r = NBS - 1; r >= 0; --r) {
+      float s = (r == t) ? 1.0f : 0.0f;
+#pragma unroll
+      for (int k = r + 1; k < NBS; ++k) s = __builtin_fmaf(-D[r][k], x[k], s);
+      x[r] = s / D[r][r];
+    }
+  }
+  float* out = (upper ? inv_u : inv_l) + (int64_t)blockIdx.x * NBS * NBS;
+#pragma unroll
+  for (int r = 0; r < NBS; ++r) out[r * NBS + t] = x[r];
+}
+
+// One block row of T X = B per workgroup (see the file comment). Lane
+// (row, cq) owns row `row` of the block and the column quarter cq (16
+// columns) of every 64 x 64 block it multiplies; the four partial sums are
+// added in LDS in a fixed order. NR: right-hand sides per launch (1 or 16).
+// sync[0] = ticket, sync[1] = error flag, sync[2] = blocks published.
+template <typename LT, bool LOWER, bool TRANS, int NR>
 __global__ void __launch_bounds__(256) trsm_kernel(int64_t n, int nrhs, const LT* __restrict__ lu,
                                                    int64_t ld, float* __restrict__ b, int64_t ldb,
+                                                   const float* __restrict__ inv,
                                                    int* __restrict__ sync, int nblk) {
-  // sync[0] = ticket counter, sync[1] = error flag, sync[2 + i] = block i done
-  __shared__ float Ts[NBS][NBS + 1];
-  __shared__ float Xs[NBS][RC];
-  __shared__ int s_ticket;
+  constexpr int CH = NR == 1 ? 8 : 2;  // solution blocks staged in LDS at a time
+  constexpr int NP = NR == 1 ? 1 : NR + 1;
+  __shared__ float Xs[CH * NBS][NR];
+  __shared__ float Red[4][NBS][NP];
+  __shared__ int s_ticket, s_avail;
   const int t = threadIdx.x;
-  const int row = t % NBS, kg = t / NBS;  // rhs kg*4 .. kg*4+3
+  const int row = t % NBS, cq = t / NBS;
   if (t == 0)
     s_ticket = __hip_atomic_fetch_add(&sync[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
   const int tk = s_ticket;
   const int bi = LOWER ? tk : nblk - 1 - tk;
-  const int64_t r0 = (int64_t)bi * NBS;
-  const int64_t gr = r0 + row;
-  float acc[4];
+  const int64_t gr = (int64_t)bi * NBS + row;
+
+  // this lane's entries of the inverted diagonal block (transposed for the
+  // transposed triangles: (T_ii)^-1 = ((D_ii)^-1)^T)
+  float dv[16];
+  {
+    const float* ib = inv + (int64_t)bi * NBS * NBS;
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    int k = kg * 4 + q;
-    acc[q] = (gr < n && k < nrhs) ? b[gr + k * ldb] : 0.0f;
-  }
-  auto load_T = [&](int bj) {
-    const int64_t c0 = (int64_t)bj * NBS;
-    // 64 x 64 block: threads walk the contiguous direction of the storage
-    for (int e = t; e < NBS * NBS; e += 256) {
-      int a = e % NBS, bq = e / NBS;
-      int rr = TRANS ? bq : a, cc = TRANS ? a : bq;
-      int64_t R = r0 + rr, Cc = c0 + cc;
-      // rows/columns past n are padded with the identity so that the
-      // substitution leaves them at zero (no 0/0 in a ragged last block)
-      Ts[rr][cc] = (R < n && Cc < n) ? tel<LT, TRANS>(lu, ld, R, Cc) : (R == Cc ? 1.0f : 0.0f);
+    for (int c = 0; c < 16; ++c) {
+      const int col = cq * 16 + c;
+      dv[c] = TRANS ? ib[col * NBS + row] : ib[row * NBS + col];
     }
-  };
-  // off-diagonal blocks, in dependency order
-  for (int s = 0; s < (LOWER ? bi : nblk - 1 - bi); ++s) {
-    const int bj = LOWER ? s : nblk - 1 - s;
-    load_T(bj);
-    if (t == 0) {
-      long long spins = 0;
-      while (__hip_atomic_load(&sync[2 + bj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-        if (++spins > kSpinLimit ||
-            __hip_atomic_load(&sync[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-          __hip_atomic_store(&sync[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
+  }
+  float acc[NR];
+#pragma unroll
+  for (int k = 0; k < NR; ++k) acc[k] = 0.0f;
+
+  const int nsteps = tk;  // blocks before this one in dependency order
+  const int64_t grc = gr < n ? gr : n - 1;  // in-bounds row for the loads
+  int avail = 0;                            // blocks known to be published
+  for (int s0 = 0; s0 < nsteps; s0 += CH) {
+    const int cnt = nsteps - s0 < CH ? nsteps - s0 : CH;
+    // this lane's entries of the chunk's blocks of T: issued before the wait
+    // (they do not depend on it), with clamped addresses and no branches so
+    // the loads are all in flight together
+    float tv[CH][16];
+#pragma unroll
+    for (int q = 0; q < CH; ++q) {
+      const int qq = q < cnt ? q : cnt - 1;
+      const int bj = LOWER ? s0 + qq : nblk - 1 - (s0 + qq);
+      const int64_t c0 = (int64_t)bj * NBS + cq * 16;
+#pragma unroll
+      for (int c = 0; c < 16; ++c) {
+        const int64_t C = c0 + c < n ? c0 + c : n - 1;
+        const float v = tel<LT, TRANS>(lu, ld, grc, C);
+        tv[q][c] = (gr < n && c0 + c < n) ? v : 0.0f;
+      }
+    }
+    // wait until blocks s0 .. s0 + cnt - 1 (in dependency order) are published
+    if (avail < s0 + cnt) {
+      if (t == 0) {
+        long long spins = 0;
+        int a = __hip_atomic_load(&sync[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while (a < s0 + cnt) {
+          if (++spins > kSpinLimit ||
+              __hip_atomic_load(&sync[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+            __hip_atomic_store(&sync[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            a = nblk;  // give up: the results are discarded by the host
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          a = __hip_atomic_load(&sync[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        __builtin_amdgcn_s_sleep(1);
+        s_avail = a;
       }
+      __syncthreads();
+      avail = s_avail;
+    }
+    // stage the published solution blocks. They were written during this
+    // launch by other workgroups with agent-scope atomic stores; agent-scope
+    // atomic loads read them at the device coherence point (no cache
+    // invalidation needed).
+    for (int e = t; e < cnt * NBS * NR; e += 256) {
+      const int rr = e % (cnt * NBS), k = e / (cnt * NBS);
+      const int q = rr / NBS, c = rr % NBS;
+      const int bj = LOWER ? s0 + q : nblk - 1 - (s0 + q);
+      const int64_t C = (int64_t)bj * NBS + c;
+      Xs[rr][k] = (C < n && k < nrhs)
+                      ? __hip_atomic_load(&b[C + k * ldb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                      : 0.0f;
     }
     __syncthreads();
-    // X_j was published with agent-coherent stores before its flag; read it
-    // with agent-coherent loads (they bypass the non-coherent caches, so no
-    // cache invalidation is needed)
-    {
-      const int64_t c0 = (int64_t)bj * NBS;
-      for (int e = t; e < NBS * RC; e += 256) {
-        int c = e % NBS, k = e / NBS;
-        int64_t C = c0 + c;
-        Xs[c][k] = (C < n && k < nrhs)
-                       ? __hip_atomic_load(&b[C + k * ldb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                       : 0.0f;
-      }
-    }
-    __syncthreads();
-#pragma unroll 8
-    for (int c = 0; c < NBS; ++c) {
-      float tv = Ts[row][c];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) acc[q] = __builtin_fmaf(-tv, Xs[c][kg * 4 + q], acc[q]);
+    for (int q = 0; q < CH; ++q) {
+      if (q < cnt) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+#pragma unroll
+          for (int k = 0; k < NR; ++k)
+            acc[k] = __builtin_fmaf(tv[q][c], Xs[q * NBS + cq * 16 + c][k], acc[k]);
+        }
+      }
     }
     __syncthreads();
   }
-  // diagonal block: substitution in LDS
-  load_T(bi);
+
+  // right-hand side of the diagonal block: B_i - sum_j T_ij X_j
+#pragma unroll
+  for (int k = 0; k < NR; ++k) Red[cq][row][k] = acc[k];
   __syncthreads();
-  for (int s = 0; s < NBS; ++s) {
-    const int c = LOWER ? s : NBS - 1 - s;
-    if (row == c) {
+  for (int k = cq; k < NR; k += 4) {
+    const float sum = ((Red[0][row][k] + Red[1][row][k]) + Red[2][row][k]) + Red[3][row][k];
+    const float bv = b[grc + (k < nrhs ? k : 0) * ldb];
+    Xs[row][k] = (gr < n && k < nrhs) ? bv - sum : 0.0f;
+  }
+  __syncthreads();
+  // X_i = (T_ii)^-1 (B_i - ...)
+  float y[NR];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        float x = acc[q];
-        if (!UNIT) x = x / Ts[c][c];
-        acc[q] = x;
-        Xs[c][kg * 4 + q] = x;
-      }
-    }
-    __syncthreads();
-    if (LOWER ? row > c : row < c) {
-      float tv = Ts[row][c];
+  for (int k = 0; k < NR; ++k) y[k] = 0.0f;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) acc[q] = __builtin_fmaf(-tv, Xs[c][kg * 4 + q], acc[q]);
-    }
+  for (int c = 0; c < 16; ++c) {
+#pragma unroll
+    for (int k = 0; k < NR; ++k) y[k] = __builtin_fmaf(dv[c], Xs[cq * 16 + c][k], y[k]);
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    int k = kg * 4 + q;
+  for (int k = 0; k < NR; ++k) Red[cq][row][k] = y[k];
+  __syncthreads();
+  for (int k = cq; k < NR; k += 4) {
+    const float xv = ((Red[0][row][k] + Red[1][row][k]) + Red[2][row][k]) + Red[3][row][k];
     if (gr < n && k < nrhs)
-      __hip_atomic_store(&b[gr + k * ldb], acc[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&b[gr + k * ldb], xv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  // the solution went out with agent-scope atomic stores, which leave no
+  // dirty cache lines to write back: waiting for their completion
+  // (s_waitcnt vmcnt(0)) orders them before the count below
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_s_waitcnt(0x0F70);
   __syncthreads();
-  if (t == 0) __hip_atomic_store(&sync[2 + bi], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  // blocks finish in dependency order: this one is number tk + 1
+  if (t == 0) __hip_atomic_store(&sync[2], tk + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void fill_int_kernel(int* p, int n, int v) {
@@ -264,30 +349,35 @@ __global__ void maxreduce_kernel(int64_t n, const double* __restrict__ v, double
   if (threadIdx.x == 0) *out = red[0];
 }
 
-template <typename LT, bool LOWER, bool UNIT, bool TRANS>
+template <typename LT, bool LOWER, bool TRANS>
 int launch_trsm(remifa_handle_t h, int64_t n, int nrhs, const void* lu, int64_t ld, float* b,
-                int64_t ldb, int* sync) {
+                int64_t ldb, const float* inv, int* sync) {
   const int nblk = (int)ceil_div(n, NBS);
-  fill_int_kernel<<<(unsigned)ceil_div(nblk + 2, 256), 256, 0, h->stream>>>(sync, nblk + 2, 0);
-  trsm_kernel<LT, LOWER, UNIT, TRANS>
-      <<<nblk, 256, 0, h->stream>>>(n, nrhs, (const LT*)lu, ld, b, ldb, sync, nblk);
+  fill_int_kernel<<<1, 256, 0, h->stream>>>(sync, 3, 0);
+  if (nrhs == 1)
+    trsm_kernel<LT, LOWER, TRANS, 1>
+        <<<nblk, 256, 0, h->stream>>>(n, nrhs, (const LT*)lu, ld, b, ldb, inv, sync, nblk);
+  else
+    trsm_kernel<LT, LOWER, TRANS, RC>
+        <<<nblk, 256, 0, h->stream>>>(n, nrhs, (const LT*)lu, ld, b, ldb, inv, sync, nblk);
   RF_HIP(hipGetLastError());
   return 0;
 }
 
 template <typename LT>
 int getrs_typed(remifa_handle_t h, int trans, int64_t n, int64_t nrhs, const void* lu,
-                int64_t ld, float* b, int64_t ldb, int* sync) {
+                int64_t ld, float* b, int64_t ldb, const float* inv_l, const float* inv_u,
+                int* sync) {
   for (int64_t k0 = 0; k0 < nrhs; k0 += RC) {
     int nr = (int)std::min<int64_t>(RC, nrhs - k0);
     float* bk = b + k0 * ldb;
     if (!trans) {
-      RF_CHECK((launch_trsm<LT, true, true, false>(h, n, nr, lu, ld, bk, ldb, sync)));
-      RF_CHECK((launch_trsm<LT, false, false, false>(h, n, nr, lu, ld, bk, ldb, sync)));
+      RF_CHECK((launch_trsm<LT, true, false>(h, n, nr, lu, ld, bk, ldb, inv_l, sync)));
+      RF_CHECK((launch_trsm<LT, false, false>(h, n, nr, lu, ld, bk, ldb, inv_u, sync)));
     } else {
-      // A^T = U^T L^T: U^T (lower, non-unit) then L^T (upper, unit)
-      RF_CHECK((launch_trsm<LT, true, false, true>(h, n, nr, lu, ld, bk, ldb, sync)));
-      RF_CHECK((launch_trsm<LT, false, true, true>(h, n, nr, lu, ld, bk, ldb, sync)));
+      // A^T = U^T L^T: U^T (lower) then L^T (upper, unit)
+      RF_CHECK((launch_trsm<LT, true, true>(h, n, nr, lu, ld, bk, ldb, inv_u, sync)));
+      RF_CHECK((launch_trsm<LT, false, true>(h, n, nr, lu, ld, bk, ldb, inv_l, sync)));
     }
   }
   return 0;
@@ -295,15 +385,51 @@ int getrs_typed(remifa_handle_t h, int trans, int64_t n, int64_t nrhs, const voi
 
 }  // namespace
 
-// sync words for one solve: ticket, error flag, one flag per block
-static size_t trsm_sync_bytes(int64_t n) { return sizeof(int) * (size_t)(ceil_div(n, NBS) + 2); }
+// solve workspace: sync words (ticket, error flag, published count), then
+// the inverted diagonal blocks of L and of U
+struct TrsmWs {
+  int* sync;
+  float* inv_l;
+  float* inv_u;
+};
 
+static size_t trsm_ws_bytes(int64_t n) {
+  return 256 + 2 * sizeof(float) * NBS * NBS * (size_t)ceil_div(n, NBS);
+}
+
+static TrsmWs trsm_ws(void* p, int64_t n) {
+  char* c = (char*)p;
+  const size_t inv = sizeof(float) * NBS * NBS * (size_t)ceil_div(n, NBS);
+  return TrsmWs{(int*)c, (float*)(c + 256), (float*)(c + 256 + inv)};
+}
+
+// invert the diagonal blocks of the stored factors (once per factorization)
+static int trsm_invert_launch(remifa_handle_t h, int lu_t, int64_t n, const void* lu, int64_t ld,
+                              const TrsmWs& w) {
+  const int nblk = (int)ceil_div(n, NBS);
+  timer_begin(h, KC_TRSM, (double)n * NBS * (lu_t == REMIFA_R_16F ? 2 : 4) * 2, 1);
+  if (lu_t == REMIFA_R_16F)
+    diag_inv_kernel<half_t><<<dim3(nblk, 2), 64, 0, h->stream>>>(n, (const half_t*)lu, ld,
+                                                                  w.inv_l, w.inv_u);
+  else
+    diag_inv_kernel<float><<<dim3(nblk, 2), 64, 0, h->stream>>>(n, (const float*)lu, ld, w.inv_l,
+                                                                 w.inv_u);
+  RF_HIP(hipGetLastError());
+  timer_end(h);
+  return 0;
+}
+
+// (L U) X = B or (L U)^T X = B in place in b; the diagonal blocks must have
+// been inverted into w
 int getrs_launch(remifa_handle_t h, int lu_t, int trans, int64_t n, int64_t nrhs, const void* lu,
-                 int64_t ld, float* b, int64_t ldb, int* sync) {
+                 int64_t ld, float* b, int64_t ldb, const TrsmWs& w) {
   if (n == 0 || nrhs == 0) return 0;
-  timer_begin(h, KC_TRSM, 2.0 * (double)n * (double)n * (double)nrhs);
-  int st = lu_t == REMIFA_R_16F ? getrs_typed<half_t>(h, trans, n, nrhs, lu, ld, b, ldb, sync)
-                                : getrs_typed<float>(h, trans, n, nrhs, lu, ld, b, ldb, sync);
+  // bytes: both triangles read once per 16 right-hand sides
+  const double es = lu_t == REMIFA_R_16F ? 2.0 : 4.0;
+  timer_begin(h, KC_TRSM, (double)n * (double)n * es * (double)ceil_div(nrhs, RC), 0);
+  int st = lu_t == REMIFA_R_16F
+               ? getrs_typed<half_t>(h, trans, n, nrhs, lu, ld, b, ldb, w.inv_l, w.inv_u, w.sync)
+               : getrs_typed<float>(h, trans, n, nrhs, lu, ld, b, ldb, w.inv_l, w.inv_u, w.sync);
   timer_end(h);
   return st;
 }
@@ -348,11 +474,12 @@ int remifa_getrs_nopiv(remifa_handle_t h, int lu_t, int trans, int64_t n, int64_
   if (!d_b && n > 0 && nrhs > 0) return -8;
   if (ldb < std::max<int64_t>(n, 1)) return -9;
   if (n == 0 || nrhs == 0) return 0;
-  RF_CHECK(ensure_workspace(h, trsm_sync_bytes(n)));
-  int* sync = (int*)h->ws;
-  RF_CHECK(getrs_launch(h, lu_t, trans, n, nrhs, d_lu, ldlu, d_b, ldb, sync));
+  RF_CHECK(ensure_workspace(h, trsm_ws_bytes(n)));
+  const TrsmWs w = trsm_ws(h->ws, n);
+  RF_CHECK(trsm_invert_launch(h, lu_t, n, d_lu, ldlu, w));
+  RF_CHECK(getrs_launch(h, lu_t, trans, n, nrhs, d_lu, ldlu, d_b, ldb, w));
   int err = 0;
-  RF_HIP(hipMemcpyAsync(&err, sync + 1, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  RF_HIP(hipMemcpyAsync(&err, w.sync + 1, sizeof(int), hipMemcpyDeviceToHost, h->stream));
   RF_HIP(hipStreamSynchronize(h->stream));
   if (err) {
     set_last_error("triangular solve: dependency wait timed out");
@@ -406,7 +533,7 @@ int remifa_ir_solve(remifa_handle_t h, int lu_t, int64_t n, int64_t nrhs, const 
   // workspace: sync words | fp32 correction (n x nrhs) | residual (n x nrhs)
   // | residual partials | column sums (n) | norms (2 nrhs + 1)
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  const size_t b_sync = al(trsm_sync_bytes(n));
+  const size_t b_sync = al(trsm_ws_bytes(n));
   const size_t b_w = al(sizeof(float) * n * nrhs);
   const size_t b_r = al(sizeof(double) * n * nrhs);
   const size_t b_part = al(resid_part_bytes(n, std::min<int64_t>(nrhs, RES_RHS)));
@@ -414,7 +541,7 @@ int remifa_ir_solve(remifa_handle_t h, int lu_t, int64_t n, int64_t nrhs, const 
   const size_t b_nrm = al(sizeof(double) * (2 * nrhs + 2));
   RF_CHECK(ensure_workspace(h, b_sync + b_w + b_r + b_part + b_cs + b_nrm));
   char* p = (char*)h->ws;
-  int* sync = (int*)p;
+  const TrsmWs tw = trsm_ws(p, n);
   float* w = (float*)(p + b_sync);
   double* r = (double*)(p + b_sync + b_w);
   double* part = (double*)(p + b_sync + b_w + b_r);
@@ -429,7 +556,8 @@ int remifa_ir_solve(remifa_handle_t h, int lu_t, int64_t n, int64_t nrhs, const 
   colmax_kernel<<<(unsigned)nrhs, 256, 0, s>>>(n, d_b, ldb, nrm + nrhs);
   // x0 = (LU)^-1 b in fp32
   d2f_kernel<<<gv, 256, 0, s>>>(n, (int)nrhs, d_b, ldb, w, n);
-  RF_CHECK(getrs_launch(h, lu_t, 0, n, nrhs, d_lu, ldlu, w, n, sync));
+  RF_CHECK(trsm_invert_launch(h, lu_t, n, d_lu, ldlu, tw));
+  RF_CHECK(getrs_launch(h, lu_t, 0, n, nrhs, d_lu, ldlu, w, n, tw));
   axpy_f2d_kernel<<<gv, 256, 0, s>>>(n, w, n, d_x, ldx, 1);
   RF_HIP(hipGetLastError());
 
@@ -444,7 +572,7 @@ int remifa_ir_solve(remifa_handle_t h, int lu_t, int64_t n, int64_t nrhs, const 
     std::vector<double> xn(nrhs);
     RF_HIP(hipMemcpyAsync(xn.data(), part, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
     int serr = 0;
-    RF_HIP(hipMemcpyAsync(&serr, sync + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    RF_HIP(hipMemcpyAsync(&serr, tw.sync + 1, sizeof(int), hipMemcpyDeviceToHost, s));
     RF_HIP(hipStreamSynchronize(s));
     if (serr) {
       set_last_error("triangular solve: dependency wait timed out");
@@ -468,7 +596,7 @@ int remifa_ir_solve(remifa_handle_t h, int lu_t, int64_t n, int64_t nrhs, const 
     }
     if (it >= maxit || worst != worst) break;
     d2f_kernel<<<gv, 256, 0, s>>>(n, (int)nrhs, r, n, w, n);
-    RF_CHECK(getrs_launch(h, lu_t, 0, n, nrhs, d_lu, ldlu, w, n, sync));
+    RF_CHECK(getrs_launch(h, lu_t, 0, n, nrhs, d_lu, ldlu, w, n, tw));
     axpy_f2d_kernel<<<gv, 256, 0, s>>>(n, w, n, d_x, ldx, 0);
     RF_HIP(hipGetLastError());
   }

@@ -1,33 +1,49 @@
 // Fused sub-panel factorization: the inner ib loop of the reference drivers
 // (src/lu_nopiv_inner_ll.hxx:197-370, src/lu_nopiv_inner_rl.hxx:104-196)
-// for an sw-wide slice (sw <= 64) of the outer block, in one launch.
+// for an sw-wide slice (sw <= 64) of the outer block, in two launches.
 //
 // Input: the sw-wide block column l[0:cm, 0:sw] (its top sw x sw tile is the
 // diagonal tile) and the sw-tall block row u[0:sw, sw:cm], already updated by
 // everything left of / above the slice. Output: the factors, in place, and
 // optionally their fp16 mirror cp (the reference's d_a_cpy).
 //
-// Every workgroup (256 lanes) factors the diagonal tile redundantly in LDS,
-// then handles 256 rows of L (one row per lane, sw values in registers) or
-// 256 columns of U (one column per lane). Inside the slice the work follows
-// the reference's ib-granular semantics:
+//  1. tile_kernel (one wavefront, the tile's rows in registers) factors the
+//     diagonal tile, padded with the identity to 64 x 64, stores it back, and
+//     publishes it to a small global table: the factors D and D^T, their operand-rounded
+//     copies (the cross-block GEMM operands) and the pivot reciprocals.
+//  2. vec_kernel (one wavefront per workgroup): every lane owns one row of L
+//     below the tile or one column of U right of it and solves it against
+//     the tile. A row x of L needs x <- x U^-1 and a column v of U needs
+//     v <- L^-1 v; with M = D for rows and M = D^T for columns (unit
+//     reciprocals for columns) both are the same recurrence over the 64
+//     positions of the lane's vector. The table is read with wave-uniform
+//     addresses (scalar loads), the lane's vector goes through registers in
+//     groups of 8 and is kept in lane-private LDS rows.
+//
+// Inside the slice the work follows the reference's ib-granular semantics:
 //  * within an ib block (the reference's panel kernel, src/lu_nopiv.cu:123-
 //    170): reciprocal of the pivot, multiply, fused multiply-subtract, in the
-//    factor precision IF (fp32, or fp16 with one rounding per operation);
+//    factor precision IF (fp32, or fp16 with one rounding per operation),
+//    each entry receiving its updates in pivot order;
 //  * across ib blocks (the reference's inner GEMMs): products of operands
 //    rounded to the update compute type (fp16 / bf16 for TC types), summed
-//    from zero (fp32, or fp16 for FP16/TC16), subtracted once and rounded to
-//    the panel precision IP -- left-looking (LL: one update per column block)
-//    or right-looking (RL: a rank-ib update after each block).
-// The diagonal tile is read by every workgroup; the last workgroup to have
-// read it (an agent-scope counter) writes it back and re-arms the counter,
-// so no snapshot copy is needed.
+//    from zero in k order (fp32, or fp16 for FP16/TC16), subtracted once and
+//    rounded to the panel precision IP -- left-looking (LL: one update per
+//    column block) or right-looking (RL: a rank-ib update after each block).
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace remifa {
 namespace {
 
-constexpr int SW = 64;  // widest sub-panel
+constexpr int SW = 64;  // tile edge (slices are at most 64 wide)
+constexpr int PD = 4;   // LDS row padding of the tile
+constexpr int G = 8;    // vector entries held in registers at a time
+
+// published tile: D, D^T, op(D), op(D)^T (64 x 64 row-major each), 1/u_kk, ones
+constexpr int T_D = 0, T_DT = 4096, T_DOP = 8192, T_DOPT = 12288, T_R = 16384, T_ONE = 16448;
+constexpr int T_FLOATS = 16512;
 
 enum SubMode {
   SM_F32 = 0,     // fp32 data, fp32 updates (I = FP32)
@@ -64,265 +80,298 @@ struct Sem {
     else
       return __builtin_fmaf(-a, b, c);
   }
-  // c - sum_j a_j b_j over the cross-block operands, rounded to IP
-  struct Acc {
-    float f = 0.0f;
-    half_t h = (half_t)0.0f;
-    __device__ __forceinline__ void add(float a, float b) {
-      if constexpr (MODE == SM_F16_H)
-        h = __builtin_fmaf16((half_t)a, (half_t)b, h);
-      else
-        f = __builtin_fmaf(op(a), op(b), f);
-    }
-    __device__ __forceinline__ float apply(float c) const {
-      if constexpr (MODE == SM_F16_H) return (float)((half_t)c - h);
-      float r = c - f;
-      return HALF ? r16(r) : r;
-    }
-  };
+  // cross-block product sums: operands are already op-rounded
+  static __device__ __forceinline__ float acc_add(float acc, float a, float b) {
+    if constexpr (MODE == SM_F16_H)
+      return (float)__builtin_fmaf16((half_t)a, (half_t)b, (half_t)acc);
+    else
+      return __builtin_fmaf(a, b, acc);
+  }
+  // c - acc, rounded to the panel precision
+  static __device__ __forceinline__ float apply(float c, float acc) {
+    if constexpr (MODE == SM_F16_H) return (float)((half_t)c - (half_t)acc);
+    float r = c - acc;
+    return HALF ? r16(r) : r;
+  }
 };
 
-template <typename T, int IB, int MODE, bool RL>
-__global__ void __launch_bounds__(256) subpanel_kernel(int64_t cm, int sw, T* __restrict__ l,
-                                                       int64_t ldl, T* __restrict__ u, int64_t ldu,
-                                                       half_t* __restrict__ cp, int64_t ldcp,
-                                                       int* counter, int nl_wg) {
+// ---------------------------------------------------------------------------
+// 1. the diagonal tile: one wavefront, lane i holds row i of the (identity-
+// padded) 64 x 64 tile in registers. At pivot k lane k's row is final; it is
+// published to LDS and every lane below reads it back as a broadcast. The
+// reads of the published rows go through LDS, so one barrier per pivot.
+// compile-time loop: f(std::integral_constant<int, B>), f(...B + S), ... < E
+template <int B, int E, int S = 1, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + S, E, S>(f);
+  }
+}
+
+template <int MODE, bool RL>
+__global__ void __launch_bounds__(64) tile_kernel(int sw, int ib, void* __restrict__ lv,
+                                                  int64_t ldl, half_t* __restrict__ cp,
+                                                  int64_t ldcp, float* __restrict__ tab) {
   using Sm = Sem<MODE>;
-  using Acc = typename Sm::Acc;
-  constexpr int NBLK = SW / IB;
-  __shared__ float D[SW][SW + 1];  // D[i][j]
-  __shared__ float R[SW];          // reciprocals of the pivots
-  __shared__ int s_last;
-  const int t = threadIdx.x;
-
-  // ---- load the diagonal tile (lane t < sw owns row t)
-  if (t < sw) {
+  using T = typename std::conditional<(MODE <= SM_F32_TC), float, half_t>::type;
+  T* const l = (T*)lv;
+  __shared__ float U[SW][SW + PD];   // published rows (row k: columns k..63)
+  __shared__ float Uo[SW][SW + PD];  // op-rounded published rows
+  __shared__ float Xo[SW][SW + 1];   // lane-private op(L) values (block sums)
+  __shared__ float Acc[SW][SW + 1];  // lane-private block sums
+  __shared__ float R[SW];            // reciprocals of the pivots
+  const int i = threadIdx.x;
+  float x[SW];
+  {
+    const int ic = i < sw ? i : sw - 1;
 #pragma unroll
-    for (int j = 0; j < SW; ++j)
-      if (j < sw) D[t][j] = (float)l[t + (int64_t)j * ldl];
-  }
-  __syncthreads();
-  if (t == 0) {
-    // every workgroup has read the tile before the last one rewrites it
-    int old = __hip_atomic_fetch_add(counter, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    s_last = (old == (int)gridDim.x - 1);
+    for (int j = 0; j < SW; ++j) {
+      const int jc = j < sw ? j : sw - 1;
+      const float v = (float)l[ic + (int64_t)jc * ldl];
+      x[j] = (i < sw && j < sw) ? v : (i == j ? 1.0f : 0.0f);
+    }
   }
 
-  // ---- factor the diagonal tile, block by block: row t % 64 is shared by
-  // four lanes, lane quarter q = t / 64 takes the columns c = q (mod 4)
-  const bool own = t < sw;
-  const int drow = t & 63, dq = t >> 6;
-  const bool down = drow < sw;
-  for (int k0 = 0; k0 < sw; k0 += IB) {
-    const int kend = k0 + IB < sw ? k0 + IB : sw;
-    if (!RL && k0 > 0) {
-      // left-looking cross-block update: columns of block kb for the rows
-      // >= k0, and the columns right of the block for the rows of the block
-      if (down && drow >= k0) {
-        const int cend = (drow < kend) ? sw : kend;
-        for (int c = k0 + dq; c < cend; c += 4) {
-          Acc acc;
-          for (int j = 0; j < k0; ++j) acc.add(D[drow][j], D[j][c]);
-          D[drow][c] = acc.apply(D[drow][c]);
+  // Blocks of ib pivots (a runtime loop). The block sums run as runtime loops
+  // into the lane's LDS row Acc; the pivots are one unrolled sequence over
+  // all 64 positions, skipped outside the current block, so the row stays in
+  // registers with fixed indices and the code exists once.
+  for (int kb = 0; kb < SW; kb += ib) {
+    const int kend = kb + ib;
+    if (!RL && kb > 0) {
+      // left-looking update at the start of the block: its columns for the
+      // rows >= kb, every column right of it for the rows of the block;
+      // products summed from zero in k order, subtracted once
+      for (int cb = kb & ~7; cb < SW; cb += 8) {
+        float acc[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[c] = 0.0f;
+#pragma unroll 2
+        for (int j = 0; j < kb; ++j) {
+          const float xj = Xo[i][j];
+          const float* ur = &Uo[j][cb];
+#pragma unroll
+          for (int c = 0; c < 8; ++c) acc[c] = Sm::acc_add(acc[c], xj, ur[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) Acc[i][cb + c] = acc[c];
+      }
+#pragma unroll
+      for (int c = 0; c < SW; ++c) {
+        if (c >= kb) {
+          if (i >= kb && (c < kend || i < kend)) x[c] = Sm::apply(x[c], Acc[i][c]);
         }
       }
-      __syncthreads();
     }
-    for (int k = k0; k < kend; ++k) {
-      const float r = Sm::rcp(D[k][k]);
-      if (t == 0) R[k] = r;
-      if (down && drow > k && dq == 0) D[drow][k] = Sm::mul(D[drow][k], r);
+#pragma unroll
+    for (int k = 0; k < SW; ++k) {
+      if (k < kb || k >= kend) continue;
+      if (i == k) {
+        // row k is final: publish it and its pivot's reciprocal
+#pragma unroll
+        for (int j = k; j < SW; ++j) {
+          U[k][j] = x[j];
+          Uo[k][j] = Sm::op(x[j]);
+        }
+        R[k] = Sm::rcp(x[k]);
+      }
       __syncthreads();
-      if (down && drow > k) {
+      if (i > k) {
+        const float lk = Sm::mul(x[k], R[k]);
+        x[k] = lk;
+        Xo[i][k] = Sm::op(lk);
+        const float* ur = &U[k][0];
         // within the block for every row; the rows of the block also carry
-        // on to the right of it (their U part)
-        const float ltk = D[drow][k];
-        const int jend = (drow < kend) ? sw : kend;
-        for (int j = k + 1 + dq; j < jend; j += 4) D[drow][j] = Sm::fnma(ltk, D[k][j], D[drow][j]);
+        // on right of it (their U part)
+#pragma unroll
+        for (int j = k + 1; j < SW; ++j)
+          if (j < kend || i < kend) x[j] = Sm::fnma(lk, ur[j], x[j]);
       }
-      __syncthreads();
     }
-    if (RL && kend < sw) {
-      // right-looking rank-ib update of the trailing part of the tile
-      if (down && drow >= kend) {
-        for (int c = kend + dq; c < sw; c += 4) {
-          Acc acc;
-          for (int k = k0; k < kend; ++k) acc.add(D[drow][k], D[k][c]);
-          D[drow][c] = acc.apply(D[drow][c]);
+    if (RL && kend < SW) {
+      // block complete: right-looking rank-ib update of the trailing part
+      for (int cb = kend; cb < SW; cb += 8) {
+        float acc[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[c] = 0.0f;
+#pragma unroll 2
+        for (int q = kb; q < kend; ++q) {
+          const float xq = Xo[i][q];
+          const float* ur = &Uo[q][cb];
+#pragma unroll
+          for (int c = 0; c < 8; ++c) acc[c] = Sm::acc_add(acc[c], xq, ur[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) Acc[i][cb + c] = acc[c];
+      }
+#pragma unroll
+      for (int c = 0; c < SW; ++c) {
+        if (c >= kend) {
+          if (i >= kend) x[c] = Sm::apply(x[c], Acc[i][c]);
         }
       }
-      __syncthreads();
     }
   }
 
-  // ---- this workgroup's rows of L or columns of U
+  // store the factors and publish the table: row i of D and op(D) (the L
+  // part op-rounded multipliers, the U part op-rounded row entries), the
+  // transposes (coalesced across lanes), the reciprocals
+  if (i < sw) {
+#pragma unroll
+    for (int j = 0; j < SW; ++j) {
+      if (j < sw) {
+        l[i + (int64_t)j * ldl] = (T)x[j];
+        if (cp) cp[i + (int64_t)j * ldcp] = (half_t)x[j];
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < SW; ++j) {
+    const float o = Sm::op(x[j]);
+    tab[T_D + i * SW + j] = x[j];
+    tab[T_DT + j * SW + i] = x[j];
+    tab[T_DOP + i * SW + j] = o;
+    tab[T_DOPT + j * SW + i] = o;
+  }
+  tab[T_R + i] = R[i];
+  tab[T_ONE + i] = 1.0f;
+}
+
+// ---------------------------------------------------------------------------
+// 2. rows of L and columns of U
+template <typename T, int MODE, bool RL>
+__global__ void __launch_bounds__(64) vec_kernel(int64_t cm, int sw, int ib, T* __restrict__ l,
+                                                 int64_t ldl, T* __restrict__ u, int64_t ldu,
+                                                 half_t* __restrict__ cp, int64_t ldcp,
+                                                 const float* __restrict__ tab, int nl_wg) {
+  using Sm = Sem<MODE>;
+  // lane-private vectors (odd stride: lane t, entry j sits in bank (t + j) % 64)
+  constexpr bool OPID = MODE != SM_F32_TC && MODE != SM_F16_BF;  // op() is the identity
+  __shared__ float Xa[64][SW + 1];                // finished values, then current ones
+  __shared__ float Xo[OPID ? 1 : 64][SW + 1];     // op-rounded finished values
+  const int t = threadIdx.x;
   const int wg = blockIdx.x;
-  if (wg < nl_wg) {
-    const int64_t row = sw + (int64_t)wg * 256 + t;
-    if (row < cm) {
-      float x[SW];
-#pragma unroll
-      for (int j = 0; j < SW; ++j) x[j] = (j < sw) ? (float)l[row + (int64_t)j * ldl] : 0.0f;
-#pragma unroll
-      for (int kb = 0; kb < NBLK; ++kb) {
-        const int k0 = kb * IB;
-        if (k0 < sw) {
-          if (!RL && k0 > 0) {
-#pragma unroll
-            for (int c = 0; c < IB; ++c) {
-              if (k0 + c < sw) {
-                Acc acc;
-#pragma unroll
-                for (int j = 0; j < SW; ++j)
-                  if (j < k0) acc.add(x[j], D[j][k0 + c]);
-                x[k0 + c] = acc.apply(x[k0 + c]);
-              }
-            }
-          }
-#pragma unroll
-          for (int kk = 0; kk < IB; ++kk) {
-            const int k = k0 + kk;
-            if (k < sw) {
-              const float lk = Sm::mul(x[k], R[k]);
-              x[k] = lk;
-#pragma unroll
-              for (int j = 0; j < SW; ++j)
-                if (j > k && j < k0 + IB && j < sw) x[j] = Sm::fnma(lk, D[k][j], x[j]);
-            }
-          }
-          if (RL && k0 + IB < sw) {
-#pragma unroll
-            for (int c = 0; c < SW; ++c) {
-              if (c >= k0 + IB && c < sw) {
-                Acc acc;
-#pragma unroll
-                for (int kk = 0; kk < IB; ++kk) acc.add(x[k0 + kk], D[k0 + kk][c]);
-                x[c] = acc.apply(x[c]);
-              }
-            }
-          }
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < SW; ++j) {
-        if (j < sw) {
-          l[row + (int64_t)j * ldl] = (T)x[j];
-          if (cp) cp[row + (int64_t)j * ldcp] = (half_t)x[j];
-        }
-      }
-    }
-  } else {
-    const int64_t col = sw + (int64_t)(wg - nl_wg) * 256 + t;
-    if (col < cm) {
-      float v[SW];
-#pragma unroll
-      for (int i = 0; i < SW; ++i) v[i] = (i < sw) ? (float)u[i + col * ldu] : 0.0f;
-#pragma unroll
-      for (int kb = 0; kb < NBLK; ++kb) {
-        const int k0 = kb * IB;
-        if (k0 < sw) {
-          if (!RL && k0 > 0) {
-#pragma unroll
-            for (int r = 0; r < IB; ++r) {
-              if (k0 + r < sw) {
-                Acc acc;
-#pragma unroll
-                for (int j = 0; j < SW; ++j)
-                  if (j < k0) acc.add(D[k0 + r][j], v[j]);
-                v[k0 + r] = acc.apply(v[k0 + r]);
-              }
-            }
-          }
-#pragma unroll
-          for (int kk = 0; kk < IB; ++kk) {
-            const int k = k0 + kk;
-            if (k < sw) {
-              const float vk = v[k];
-#pragma unroll
-              for (int i = 0; i < SW; ++i)
-                if (i > k && i < k0 + IB && i < sw) v[i] = Sm::fnma(D[i][k], vk, v[i]);
-            }
-          }
-          if (RL && k0 + IB < sw) {
-#pragma unroll
-            for (int i = 0; i < SW; ++i) {
-              if (i >= k0 + IB && i < sw) {
-                Acc acc;
-#pragma unroll
-                for (int kk = 0; kk < IB; ++kk) acc.add(D[i][k0 + kk], v[k0 + kk]);
-                v[i] = acc.apply(v[i]);
-              }
-            }
-          }
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < SW; ++i) {
-        if (i < sw) {
-          u[i + col * ldu] = (T)v[i];
-          if (cp) cp[i + col * ldcp] = (half_t)v[i];
-        }
-      }
-    }
-  }
+  const bool is_l = wg < nl_wg;  // wave-uniform
+  const int64_t idx = sw + (int64_t)(is_l ? wg : wg - nl_wg) * 64 + t;
+  if (idx >= cm) return;  // no barriers below: lanes are independent
+  // entry j of the lane's vector is p[j * st]; its mirror q[j * sq]
+  T* const p = is_l ? l + idx : u + idx * ldu;
+  const int64_t st = is_l ? ldl : 1;
+  half_t* const q = cp ? (is_l ? cp + idx : cp + idx * ldcp) : nullptr;
+  const int64_t sq = is_l ? ldcp : 1;
+  const float* __restrict__ M = tab + (is_l ? T_D : T_DT);
+  const float* __restrict__ Mo = tab + (is_l ? T_DOP : T_DOPT);
+  const float* __restrict__ Rr = tab + (is_l ? T_R : T_ONE);
+  float* const xa = &Xa[t][0];
+  float* const xo = OPID ? xa : &Xo[t][0];
 
-  // ---- the last reader of the diagonal tile stores it and re-arms the counter
-  __syncthreads();
-  if (s_last) {
-    if (own) {
+  // the whole vector in one go (all loads in flight together, addresses
+  // clamped in bounds); xa[j] is overwritten by the finished value later
 #pragma unroll
-      for (int j = 0; j < SW; ++j) {
-        if (j < sw) {
-          l[t + (int64_t)j * ldl] = (T)D[t][j];
-          if (cp) cp[t + (int64_t)j * ldcp] = (half_t)D[t][j];
+  for (int j = 0; j < SW; ++j) {
+    const int jc = j < sw ? j : sw - 1;
+    const float v = (float)p[jc * st];
+    xa[j] = j < sw ? v : 0.0f;
+  }
+  for (int c0 = 0; c0 < sw; c0 += G) {
+    const int kb = c0 - c0 % ib;  // start of the ib block holding c0
+    float xb[G];
+#pragma unroll
+    for (int c = 0; c < G; ++c) xb[c] = xa[c0 + c];
+    if (!RL && kb > 0) {
+      // left-looking cross-block update, products summed from zero
+      float acc[G];
+#pragma unroll
+      for (int c = 0; c < G; ++c) acc[c] = 0.0f;
+#pragma unroll 4
+      for (int j = 0; j < kb; ++j) {
+        const float xj = xo[j];
+        const float* mr = Mo + j * SW + c0;
+#pragma unroll
+        for (int c = 0; c < G; ++c) acc[c] = Sm::acc_add(acc[c], xj, mr[c]);
+      }
+#pragma unroll
+      for (int c = 0; c < G; ++c) xb[c] = Sm::apply(xb[c], acc[c]);
+    }
+    // pivots of this ib block already finished (earlier groups), in order
+#pragma unroll 2
+    for (int k = kb; k < c0; ++k) {
+      const float lk = xa[k];
+      const float* mr = M + k * SW + c0;
+#pragma unroll
+      for (int c = 0; c < G; ++c) xb[c] = Sm::fnma(lk, mr[c], xb[c]);
+    }
+    // pivots of this group
+#pragma unroll
+    for (int kk = 0; kk < G; ++kk) {
+      const float lk = Sm::mul(xb[kk], Rr[c0 + kk]);
+      xb[kk] = lk;
+      const float* mr = M + (c0 + kk) * SW + c0;
+#pragma unroll
+      for (int c = kk + 1; c < G; ++c) xb[c] = Sm::fnma(lk, mr[c], xb[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < G; ++c) {
+      if (c0 + c < sw) {
+        p[(c0 + c) * st] = (T)xb[c];
+        if (q) q[(c0 + c) * sq] = (half_t)xb[c];
+      }
+      xa[c0 + c] = xb[c];
+      if (!OPID) xo[c0 + c] = Sm::op(xb[c]);
+    }
+    const int kend = kb + ib < sw ? kb + ib : sw;
+    if (RL && c0 + G >= kend && kend < sw) {
+      // the ib block is complete: rank-ib update of the entries right of it
+      for (int c1 = kend; c1 < sw; c1 += G) {
+        float acc[G];
+#pragma unroll
+        for (int c = 0; c < G; ++c) acc[c] = 0.0f;
+#pragma unroll 4
+        for (int k = kb; k < kend; ++k) {
+          const float xk = xo[k];
+          const float* mr = Mo + k * SW + c1;
+#pragma unroll
+          for (int c = 0; c < G; ++c) acc[c] = Sm::acc_add(acc[c], xk, mr[c]);
         }
+#pragma unroll
+        for (int c = 0; c < G; ++c) xa[c1 + c] = Sm::apply(xa[c1 + c], acc[c]);
       }
     }
-    if (t == 0) __hip_atomic_store(counter, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
 template <typename T, int MODE, bool RL>
-int launch_ib(remifa_handle_t h, int ib, int64_t cm, int sw, void* l, int64_t ldl, void* u,
-              int64_t ldu, half_t* cp, int64_t ldcp, int* counter) {
-  const int64_t rest = cm - sw;
-  const int nl = (int)ceil_div(rest > 0 ? rest : 0, 256);
-  const int nu = nl;
-  const int grid = (nl + nu) > 0 ? nl + nu : 1;
-  const int nlw = nl > 0 ? nl : 1;  // with no rows the single workgroup takes the L role
-  switch (ib) {
-    case 8:
-      subpanel_kernel<T, 8, MODE, RL><<<grid, 256, 0, h->stream>>>(cm, sw, (T*)l, ldl, (T*)u, ldu,
-                                                                    cp, ldcp, counter, nlw);
-      break;
-    case 16:
-      subpanel_kernel<T, 16, MODE, RL><<<grid, 256, 0, h->stream>>>(cm, sw, (T*)l, ldl, (T*)u, ldu,
-                                                                     cp, ldcp, counter, nlw);
-      break;
-    case 32:
-      subpanel_kernel<T, 32, MODE, RL><<<grid, 256, 0, h->stream>>>(cm, sw, (T*)l, ldl, (T*)u, ldu,
-                                                                     cp, ldcp, counter, nlw);
-      break;
-    case 64:
-      subpanel_kernel<T, 64, MODE, RL><<<grid, 256, 0, h->stream>>>(cm, sw, (T*)l, ldl, (T*)u, ldu,
-                                                                     cp, ldcp, counter, nlw);
-      break;
-    default: return REMIFA_ERR_UNSUPPORTED;
-  }
+int launch_typed(remifa_handle_t h, int ib, int64_t cm, int sw, void* l, int64_t ldl, void* u,
+                 int64_t ldu, half_t* cp, int64_t ldcp, float* tab, double bytes) {
+  timer_begin(h, KC_PANEL, (double)sw * sw * 2 * sizeof(T), 0);
+  tile_kernel<MODE, RL><<<1, 64, 0, h->stream>>>(sw, ib, l, ldl, cp, ldcp, tab);
   RF_HIP(hipGetLastError());
+  timer_end(h);
+  const int64_t rest = cm - sw;
+  if (rest <= 0) return 0;
+  const int nl = (int)ceil_div(rest, 64);
+  timer_begin(h, KC_PANEL, bytes, 1);
+  vec_kernel<T, MODE, RL><<<2 * nl, 64, 0, h->stream>>>(cm, sw, ib, (T*)l, ldl, (T*)u, ldu, cp,
+                                                         ldcp, tab, nl);
+  RF_HIP(hipGetLastError());
+  timer_end(h);
   return 0;
 }
 
 template <int MODE, bool RL>
 int launch_mode(remifa_handle_t h, int ib, int64_t cm, int sw, void* l, int64_t ldl, void* u,
-                int64_t ldu, half_t* cp, int64_t ldcp, int* counter) {
+                int64_t ldu, half_t* cp, int64_t ldcp, float* tab, double bytes) {
   if constexpr (MODE <= SM_F32_TC)
-    return launch_ib<float, MODE, RL>(h, ib, cm, sw, l, ldl, u, ldu, cp, ldcp, counter);
+    return launch_typed<float, MODE, RL>(h, ib, cm, sw, l, ldl, u, ldu, cp, ldcp, tab, bytes);
   else
-    return launch_ib<half_t, MODE, RL>(h, ib, cm, sw, l, ldl, u, ldu, cp, ldcp, counter);
+    return launch_typed<half_t, MODE, RL>(h, ib, cm, sw, l, ldl, u, ldu, cp, ldcp, tab, bytes);
 }
 
 }  // namespace
+
+// bytes of the published-tile table a slice needs in device memory
+size_t subpanel_table_bytes() { return (size_t)T_FLOATS * sizeof(float); }
 
 // mode from (storage, inner compute type): see SubMode
 int subpanel_mode(int T, int I) {
@@ -333,27 +382,25 @@ int subpanel_mode(int T, int I) {
 }
 
 int subpanel_launch(remifa_handle_t h, int T, int I, bool rl, int ib, int64_t cm, int sw, void* l,
-                    int64_t ldl, void* u, int64_t ldu, half_t* cp, int64_t ldcp, int* counter) {
+                    int64_t ldl, void* u, int64_t ldu, half_t* cp, int64_t ldcp, float* tab) {
   if (sw <= 0) return 0;
-  if (sw > SW || (ib != 8 && ib != 16 && ib != 32 && ib != 64)) return REMIFA_ERR_UNSUPPORTED;
+  if (sw > SW || (ib != 8 && ib != 16 && ib != 32 && ib != 64) || cm < sw)
+    return REMIFA_ERR_UNSUPPORTED;
   const size_t es = dtype_size(T);
-  double bytes = 2.0 * ((double)cm * sw + (double)sw * (double)(cm > sw ? cm - sw : 0)) * es;
-  timer_begin(h, KC_PANEL, bytes, sw);
+  const double rest = (double)(cm - sw);
+  const double bytes = 2.0 * (2.0 * rest * sw) * es + (cp ? 2.0 * rest * sw * 2 : 0.0);
   const int mode = subpanel_mode(T, I);
-  int st;
 #define RF_SUB(M)                                                                              \
-  st = rl ? launch_mode<M, true>(h, ib, cm, sw, l, ldl, u, ldu, cp, ldcp, counter)             \
-          : launch_mode<M, false>(h, ib, cm, sw, l, ldl, u, ldu, cp, ldcp, counter)
+  return rl ? launch_mode<M, true>(h, ib, cm, sw, l, ldl, u, ldu, cp, ldcp, tab, bytes)        \
+            : launch_mode<M, false>(h, ib, cm, sw, l, ldl, u, ldu, cp, ldcp, tab, bytes)
   switch (mode) {
-    case SM_F32: RF_SUB(SM_F32); break;
-    case SM_F32_TC: RF_SUB(SM_F32_TC); break;
-    case SM_F16_TC: RF_SUB(SM_F16_TC); break;
-    case SM_F16_H: RF_SUB(SM_F16_H); break;
-    default: RF_SUB(SM_F16_BF); break;
+    case SM_F32: RF_SUB(SM_F32);
+    case SM_F32_TC: RF_SUB(SM_F32_TC);
+    case SM_F16_TC: RF_SUB(SM_F16_TC);
+    case SM_F16_H: RF_SUB(SM_F16_H);
+    default: RF_SUB(SM_F16_BF);
   }
 #undef RF_SUB
-  timer_end(h);
-  return st;
 }
 
 }  // namespace remifa
