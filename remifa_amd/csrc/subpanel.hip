@@ -7,7 +7,7 @@
 // everything left of / above the slice. Output: the factors, in place, and
 // optionally their fp16 mirror cp (the reference's d_a_cpy).
 //
-//  1. tile_kernel (one wavefront, the tile's rows in registers) factors the
+//  1. tile_kernel (eight wavefronts, the tile in registers) factors the
 //     diagonal tile, padded with the identity to 64 x 64, stores it back, and
 //     publishes it to a small global table: the factors D and D^T, their operand-rounded
 //     copies (the cross-block GEMM operands) and the pivot reciprocals.
@@ -96,146 +96,176 @@ struct Sem {
 };
 
 // ---------------------------------------------------------------------------
-// 1. the diagonal tile: one wavefront, lane i holds row i of the (identity-
-// padded) 64 x 64 tile in registers. At pivot k lane k's row is final; it is
-// published to LDS and every lane below reads it back as a broadcast. The
-// reads of the published rows go through LDS, so one barrier per pivot.
-// compile-time loop: f(std::integral_constant<int, B>), f(...B + S), ... < E
-template <int B, int E, int S = 1, typename F>
-__device__ __forceinline__ void static_for(F&& f) {
-  if constexpr (B < E) {
-    f(std::integral_constant<int, B>{});
-    static_for<B + S, E, S>(f);
-  }
+// 1. the diagonal tile, blocked by 8-column panels: eight wavefronts, lane i
+// holds row i, wavefront w owns columns 8w..8w+7 in registers. Stage g:
+// wavefront g factors its panel (8 pivots, broadcasts inside the wavefront
+// only) and shares the multipliers through LDS; after one barrier every
+// wavefront right of it applies the panel's 8 pivots to its own columns in
+// pivot order. Every entry receives the same sequence of fused updates as in
+// a pivot-by-pivot elimination. ib (a multiple of 8) decides where the block
+// boundaries fall: at a block boundary the block sums (LL: before the block,
+// RL: after it) run as runtime loops over LDS copies of the operands.
+constexpr int TP = 8;  // panel width = columns per wavefront
+
+struct TileLds {
+  float Lp[2][SW][TP + 1];  // multipliers of the current panel (double-buffered)
+  float Bc[SW / TP][TP + 1];// per wavefront: the pivot row of its columns (+ 1/u_kk)
+  float Uo[SW][SW + PD];    // op-rounded rows of U (row k published at pivot k)
+  float Xo[SW][SW + 1];     // op-rounded multipliers
+  float R[SW];              // reciprocals of the pivots
+};
+
+// LDS writes of this wavefront complete and stay ordered before its reads
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 template <int MODE, bool RL>
-__global__ void __launch_bounds__(64) tile_kernel(int sw, int ib, void* __restrict__ lv,
-                                                  int64_t ldl, half_t* __restrict__ cp,
-                                                  int64_t ldcp, float* __restrict__ tab) {
+__global__ void __launch_bounds__(512) tile_kernel(int sw, int ib, void* __restrict__ lv,
+                                                   int64_t ldl, half_t* __restrict__ cp,
+                                                   int64_t ldcp, float* __restrict__ tab) {
   using Sm = Sem<MODE>;
   using T = typename std::conditional<(MODE <= SM_F32_TC), float, half_t>::type;
   T* const l = (T*)lv;
-  __shared__ float U[SW][SW + PD];   // published rows (row k: columns k..63)
-  __shared__ float Uo[SW][SW + PD];  // op-rounded published rows
-  __shared__ float Xo[SW][SW + 1];   // lane-private op(L) values (block sums)
-  __shared__ float Acc[SW][SW + 1];  // lane-private block sums
-  __shared__ float R[SW];            // reciprocals of the pivots
-  const int i = threadIdx.x;
-  float x[SW];
+  __shared__ TileLds S;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);  // wavefront
+  const int i = threadIdx.x % 64;                                  // row
+  const int cb = w * TP;                                           // first column
+  float x[TP];
   {
     const int ic = i < sw ? i : sw - 1;
 #pragma unroll
-    for (int j = 0; j < SW; ++j) {
+    for (int jj = 0; jj < TP; ++jj) {
+      const int j = cb + jj;
       const int jc = j < sw ? j : sw - 1;
       const float v = (float)l[ic + (int64_t)jc * ldl];
-      x[j] = (i < sw && j < sw) ? v : (i == j ? 1.0f : 0.0f);
+      x[jj] = (i < sw && j < sw) ? v : (i == j ? 1.0f : 0.0f);
     }
   }
 
-  // Blocks of ib pivots (a runtime loop). The block sums run as runtime loops
-  // into the lane's LDS row Acc; the pivots are one unrolled sequence over
-  // all 64 positions, skipped outside the current block, so the row stays in
-  // registers with fixed indices and the code exists once.
-  for (int kb = 0; kb < SW; kb += ib) {
+  for (int g = 0; g < SW / TP; ++g) {
+    const int p0 = g * TP;         // first pivot of the panel
+    const int kb = p0 - p0 % ib;   // ib block holding it: [kb, kend)
     const int kend = kb + ib;
-    if (!RL && kb > 0) {
-      // left-looking update at the start of the block: its columns for the
+    // (every wavefront takes every stage: the barrier below needs all eight;
+    // columns left of the panel are final and take no work)
+    if (!RL && p0 == kb && kb > 0 && w >= g) {
+      // left-looking update at the start of a block: its columns for the
       // rows >= kb, every column right of it for the rows of the block;
       // products summed from zero in k order, subtracted once
-      for (int cb = kb & ~7; cb < SW; cb += 8) {
-        float acc[8];
+      float acc[TP];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) acc[c] = 0.0f;
+      for (int jj = 0; jj < TP; ++jj) acc[jj] = 0.0f;
 #pragma unroll 2
-        for (int j = 0; j < kb; ++j) {
-          const float xj = Xo[i][j];
-          const float* ur = &Uo[j][cb];
+      for (int j = 0; j < kb; ++j) {
+        const float xj = S.Xo[i][j];
+        const float* ur = &S.Uo[j][cb];
 #pragma unroll
-          for (int c = 0; c < 8; ++c) acc[c] = Sm::acc_add(acc[c], xj, ur[c]);
-        }
-#pragma unroll
-        for (int c = 0; c < 8; ++c) Acc[i][cb + c] = acc[c];
+        for (int jj = 0; jj < TP; ++jj) acc[jj] = Sm::acc_add(acc[jj], xj, ur[jj]);
       }
+      if (i >= kb && (cb < kend || i < kend)) {
 #pragma unroll
-      for (int c = 0; c < SW; ++c) {
-        if (c >= kb) {
-          if (i >= kb && (c < kend || i < kend)) x[c] = Sm::apply(x[c], Acc[i][c]);
-        }
+        for (int jj = 0; jj < TP; ++jj) x[jj] = Sm::apply(x[jj], acc[jj]);
       }
     }
+    if (w == g) {
+      // factor the panel: pivot by pivot inside this wavefront
 #pragma unroll
-    for (int k = 0; k < SW; ++k) {
-      if (k < kb || k >= kend) continue;
-      if (i == k) {
-        // row k is final: publish it and its pivot's reciprocal
+      for (int kk = 0; kk < TP; ++kk) {
+        const int k = p0 + kk;
+        if (i == k) {
 #pragma unroll
-        for (int j = k; j < SW; ++j) {
-          U[k][j] = x[j];
-          Uo[k][j] = Sm::op(x[j]);
+          for (int jj = kk; jj < TP; ++jj) {
+            S.Bc[w][jj] = x[jj];
+            S.Uo[k][cb + jj] = Sm::op(x[jj]);
+          }
+          const float r = Sm::rcp(x[kk]);
+          S.Bc[w][TP] = r;
+          S.R[k] = r;
         }
-        R[k] = Sm::rcp(x[k]);
-      }
-      __syncthreads();
-      if (i > k) {
-        const float lk = Sm::mul(x[k], R[k]);
-        x[k] = lk;
-        Xo[i][k] = Sm::op(lk);
-        const float* ur = &U[k][0];
-        // within the block for every row; the rows of the block also carry
-        // on right of it (their U part)
+        wave_lds_sync();
+        if (i > k) {
+          const float lk = Sm::mul(x[kk], S.Bc[w][TP]);
+          x[kk] = lk;
+          S.Lp[g & 1][i][kk] = lk;
+          S.Xo[i][k] = Sm::op(lk);
 #pragma unroll
-        for (int j = k + 1; j < SW; ++j)
-          if (j < kend || i < kend) x[j] = Sm::fnma(lk, ur[j], x[j]);
+          for (int jj = kk + 1; jj < TP; ++jj) x[jj] = Sm::fnma(lk, S.Bc[w][jj], x[jj]);
+        }
+        wave_lds_sync();
       }
     }
-    if (RL && kend < SW) {
-      // block complete: right-looking rank-ib update of the trailing part
-      for (int cb = kend; cb < SW; cb += 8) {
-        float acc[8];
+    __syncthreads();
+    if (w > g) {
+      // the panel's pivots on this wavefront's columns, in pivot order: for
+      // every row inside the block, for the rows below it only within it
+      const bool upd = i < kend || cb < kend;
 #pragma unroll
-        for (int c = 0; c < 8; ++c) acc[c] = 0.0f;
+      for (int kk = 0; kk < TP; ++kk) {
+        const int k = p0 + kk;
+        if (i == k) {
+#pragma unroll
+          for (int jj = 0; jj < TP; ++jj) {
+            S.Bc[w][jj] = x[jj];
+            S.Uo[k][cb + jj] = Sm::op(x[jj]);
+          }
+        }
+        wave_lds_sync();
+        if (i > k && upd) {
+          const float lk = S.Lp[g & 1][i][kk];
+#pragma unroll
+          for (int jj = 0; jj < TP; ++jj) x[jj] = Sm::fnma(lk, S.Bc[w][jj], x[jj]);
+        }
+        wave_lds_sync();
+      }
+      if (RL && p0 + TP == kend && cb >= kend) {
+        // block complete: right-looking rank-ib update of the trailing part
+        float acc[TP];
+#pragma unroll
+        for (int jj = 0; jj < TP; ++jj) acc[jj] = 0.0f;
 #pragma unroll 2
         for (int q = kb; q < kend; ++q) {
-          const float xq = Xo[i][q];
-          const float* ur = &Uo[q][cb];
+          const float xq = S.Xo[i][q];
+          const float* ur = &S.Uo[q][cb];
 #pragma unroll
-          for (int c = 0; c < 8; ++c) acc[c] = Sm::acc_add(acc[c], xq, ur[c]);
+          for (int jj = 0; jj < TP; ++jj) acc[jj] = Sm::acc_add(acc[jj], xq, ur[jj]);
         }
+        if (i >= kend) {
 #pragma unroll
-        for (int c = 0; c < 8; ++c) Acc[i][cb + c] = acc[c];
-      }
-#pragma unroll
-      for (int c = 0; c < SW; ++c) {
-        if (c >= kend) {
-          if (i >= kend) x[c] = Sm::apply(x[c], Acc[i][c]);
+          for (int jj = 0; jj < TP; ++jj) x[jj] = Sm::apply(x[jj], acc[jj]);
         }
       }
     }
   }
 
-  // store the factors and publish the table: row i of D and op(D) (the L
-  // part op-rounded multipliers, the U part op-rounded row entries), the
-  // transposes (coalesced across lanes), the reciprocals
+  // store the factors and publish the table: D, op(D) (multipliers and row
+  // entries op-rounded), the transposes, the reciprocals
   if (i < sw) {
 #pragma unroll
-    for (int j = 0; j < SW; ++j) {
+    for (int jj = 0; jj < TP; ++jj) {
+      const int j = cb + jj;
       if (j < sw) {
-        l[i + (int64_t)j * ldl] = (T)x[j];
-        if (cp) cp[i + (int64_t)j * ldcp] = (half_t)x[j];
+        l[i + (int64_t)j * ldl] = (T)x[jj];
+        if (cp) cp[i + (int64_t)j * ldcp] = (half_t)x[jj];
       }
     }
   }
 #pragma unroll
-  for (int j = 0; j < SW; ++j) {
-    const float o = Sm::op(x[j]);
-    tab[T_D + i * SW + j] = x[j];
-    tab[T_DT + j * SW + i] = x[j];
+  for (int jj = 0; jj < TP; ++jj) {
+    const int j = cb + jj;
+    const float o = Sm::op(x[jj]);
+    tab[T_D + i * SW + j] = x[jj];
+    tab[T_DT + j * SW + i] = x[jj];
     tab[T_DOP + i * SW + j] = o;
     tab[T_DOPT + j * SW + i] = o;
   }
-  tab[T_R + i] = R[i];
-  tab[T_ONE + i] = 1.0f;
+  __syncthreads();
+  if (w == 0) {
+    tab[T_R + i] = S.R[i];
+    tab[T_ONE + i] = 1.0f;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -345,7 +375,7 @@ template <typename T, int MODE, bool RL>
 int launch_typed(remifa_handle_t h, int ib, int64_t cm, int sw, void* l, int64_t ldl, void* u,
                  int64_t ldu, half_t* cp, int64_t ldcp, float* tab, double bytes) {
   timer_begin(h, KC_PANEL, (double)sw * sw * 2 * sizeof(T), 0);
-  tile_kernel<MODE, RL><<<1, 64, 0, h->stream>>>(sw, ib, l, ldl, cp, ldcp, tab);
+  tile_kernel<MODE, RL><<<1, 512, 0, h->stream>>>(sw, ib, l, ldl, cp, ldcp, tab);
   RF_HIP(hipGetLastError());
   timer_end(h);
   const int64_t rest = cm - sw;
