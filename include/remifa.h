@@ -91,8 +91,9 @@ int remifa_lu_workspace_size(remifa_handle_t handle, int algo, const remifa_lu_c
  * split-K factor of the left-looking updates. */
 int remifa_set_tuning(remifa_handle_t handle, int panel_width, int split_k);
 /* Force the output tile of the update GEMMs (-1 = chosen per shape):
- * 0: 256x32, 1: 32x256, 2: 256x64, 3: 64x256, 4: 256x256, 5: 128x128
- * (fp32 operands only take 0, 1, 5). Used by the autotuner. */
+ * 0: 256x32, 1: 32x256, 2: 256x64, 3: 64x256, 4: 256x256, 5: 128x128,
+ * 6: 256x128, 7: 128x256 (fp32 operands only take 0, 1, 5). Used by the
+ * autotuner. */
 int remifa_set_gemm_tile(remifa_handle_t handle, int tile);
 const char *remifa_last_error(void);
 

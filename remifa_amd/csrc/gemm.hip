@@ -101,9 +101,10 @@ __global__ void __launch_bounds__(256) gemm_h16_kernel(int64_t M, int64_t N, int
 }
 
 // K slab depth: 16 for fp32 operands; for 16-bit operands 64, except the
-// 256x256 tile (32: its 256 accumulators leave no room for deeper prefetch)
+// tiles of 256 x 128 and up (32: their accumulators leave no room for a
+// deeper prefetch, and two 64-deep slabs would not fit the LDS)
 constexpr int slab_k(int opk, int bm, int bn) {
-  return opk == OPK_F32 ? 16 : (bm * bn >= 256 * 256 ? 32 : 64);
+  return opk == OPK_F32 ? 16 : (bm * bn >= 256 * 128 ? 32 : 64);
 }
 
 template <int OPK, typename OutT, int BM, int BN, int WM, int WN>
@@ -127,17 +128,26 @@ int launch_mfma(remifa_handle_t h, GemmArgs g, int split, bool vec) {
 
 // tile shapes (BM x BN); the timer's variant code of a launch is
 // opk << 8 | fp16-output << 4 | tile
-enum Tile { T256x32 = 0, T32x256 = 1, T256x64 = 2, T64x256 = 3, T256x256 = 4, T128x128 = 5 };
+enum Tile {
+  T256x32 = 0,
+  T32x256 = 1,
+  T256x64 = 2,
+  T64x256 = 3,
+  T256x256 = 4,
+  T128x128 = 5,
+  T256x128 = 6,
+  T128x256 = 7
+};
 
-// 256x256 for large outputs; narrow shapes (N or M = ib .. 64) use 256x32 /
-// 256x64 and their transposes so that skinny updates keep the chip busy.
+// 128x128 for large outputs (the measured best of the square tiles); narrow
+// shapes (N or M = ib .. 64) use 256x32 / 256x64 and their transposes so that
+// skinny updates keep the chip busy.
 int choose_tile(int opk, int64_t M, int64_t N) {
   if (N <= 32) return T256x32;
   if (M <= 32) return T32x256;
   if (opk == OPK_F32) return T128x128;
   if (N <= 64) return T256x64;
   if (M <= 64) return T64x256;
-  if (M * N >= (int64_t)256 * 256 * 512) return T256x256;
   return T128x128;
 }
 
@@ -153,6 +163,8 @@ int dispatch_tiles(remifa_handle_t h, int tile, GemmArgs g, int split, bool vec)
       case T256x64: return launch_mfma<OPK, OutT, 256, 64, 4, 1>(h, g, split, vec);
       case T64x256: return launch_mfma<OPK, OutT, 64, 256, 1, 4>(h, g, split, vec);
       case T256x256: return launch_mfma<OPK, OutT, 256, 256, 2, 2>(h, g, split, vec);
+      case T256x128: return launch_mfma<OPK, OutT, 256, 128, 2, 2>(h, g, split, vec);
+      case T128x256: return launch_mfma<OPK, OutT, 128, 256, 2, 2>(h, g, split, vec);
     }
   }
   return REMIFA_ERR_UNSUPPORTED;

@@ -17,8 +17,7 @@
 //    Other shapes use the element-wise VEC = false instantiation.
 //  * A slabs are stored m-contiguous ([k][m]) and B slabs k-contiguous
 //    ([n][k]), both with 16-byte stores; a B fragment is one 16-byte LDS
-//    read, an A fragment is gathered element-wise (the row padding keeps the
-//    two half-waves on different banks).
+//    read, an A fragment (fp16/bf16) two LDS transpose reads.
 //  * Workgroup ids are remapped so that every XCD (private L2) processes a
 //    contiguous range of tiles.
 
@@ -27,6 +26,18 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef short v4i16 __attribute__((ext_vector_type(4)));
+typedef uint16_t v4u16 __attribute__((ext_vector_type(4)));
+
+// gfx950 LDS transpose read (ds_read_b64_tr_b16). Within each group of 16
+// lanes, source lane L supplies an 8-byte address; result lane j receives
+// element j % 4 of the addresses of lanes j / 4, j / 4 + 4, j / 4 + 8 and
+// j / 4 + 12 (in that order).
+__device__ __forceinline__ v4u16 ds_read_tr16(const uint16_t* p) {
+  const v4i16 r = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+      (__attribute__((address_space(3))) v4i16*)(p));
+  return __builtin_bit_cast(v4u16, r);
+}
 
 enum { OPK_F16 = 0, OPK_BF16 = 1, OPK_F32 = 2 };
 
@@ -245,8 +256,16 @@ __global__ void __launch_bounds__(256) gemm_mfma_kernel(GemmArgs g) {
           xb[j] = *(const u16x8*)&bs[(wn * WTN + j * 32 + l32) * SB + ks + 8 * lh];
 #pragma unroll
         for (int i = 0; i < FM; ++i) {
-#pragma unroll
-          for (int v = 0; v < 8; ++v) ya[i][v] = as[(ks + 8 * lh + v) * SA + wm * WTM + i * 32 + l32];
+          // A^T fragment (lane: m = l32, k = 8 lh .. 8 lh + 7) from the
+          // m-contiguous slab with two LDS transpose reads: in each 16-lane
+          // group, source lane L points at 4 m-values of row k = kb + L / 4,
+          // and result lane j receives A[mb' + j][kb .. kb + 3]
+          const int grp = lane >> 4, sl = lane & 15;
+          const int mb = wm * WTM + i * 32 + 16 * (grp & 1) + 4 * (sl & 3);
+          const int kb = ks + 8 * (grp >> 1) + (sl >> 2);
+          const v4u16 lo = ds_read_tr16(&as[kb * SA + mb]);
+          const v4u16 hi = ds_read_tr16(&as[(kb + 4) * SA + mb]);
+          ya[i] = (u16x8){lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
         }
 #pragma unroll
         for (int i = 0; i < FM; ++i)

@@ -135,7 +135,7 @@ int remifa_set_tuning(remifa_handle_t h, int panel_width, int split_k) {
 
 int remifa_set_gemm_tile(remifa_handle_t h, int tile) {
   if (!h) return -1;
-  if (tile < -1 || tile > 5) return -2;
+  if (tile < -1 || tile > 7) return -2;
   h->gemm_tile = tile;
   return 0;
 }

@@ -15,7 +15,7 @@
 // error and exits, so the grid always drains. The factors are read once
 // (HBM-bound).
 //
-// Residual R = B - A X in FP64: 2-D grid over (256-row block, 1024-column
+// Residual R = B - A X in FP64: 2-D grid over (256-row block, 512-column
 // chunk), fp64 fma per element, per-chunk partials summed in chunk order by a
 // second kernel (deterministic).
 #include <algorithm>
@@ -233,39 +233,64 @@ __global__ void fill_int_kernel(int* p, int n, int v) {
 }
 
 // ---------------------------------------------------------------- residual
-constexpr int RES_ROWS = 256;
-constexpr int RES_COLS = 512;
-constexpr int RES_RHS = 16;
+constexpr int RES_ROWS = 256;  // rows per workgroup (one per lane)
+constexpr int RES_COLS = 512;  // columns per chunk
+constexpr int RES_RHS = 16;    // right-hand sides per launch
 
+// part[chunk][rhs][n] = sum over the chunk's columns j of a_ij x_jk, in FP64.
+// NR = 1: four interleaved accumulators (j mod 4) combined in a fixed order,
+// so several loads are in flight per lane; NR = 16: one accumulator per
+// right-hand side (16 FMAs per element of A).
+template <int NR>
 __global__ void __launch_bounds__(256) resid_partial_kernel(int64_t n, int nrhs,
                                                             const float* __restrict__ a,
                                                             int64_t lda,
                                                             const double* __restrict__ x,
                                                             int64_t ldx, double* __restrict__ part) {
-  // part[chunk][rhs][n]
-  __shared__ double xs[RES_COLS][RES_RHS];
+  __shared__ double xs[RES_COLS][NR];
   const int64_t i = (int64_t)blockIdx.x * RES_ROWS + threadIdx.x;
   const int64_t j0 = (int64_t)blockIdx.y * RES_COLS;
-  const int64_t j1 = std::min<int64_t>(j0 + RES_COLS, n);
-  for (int e = threadIdx.x; e < RES_COLS * RES_RHS; e += 256) {
+  for (int e = threadIdx.x; e < RES_COLS * NR; e += 256) {
     int c = e % RES_COLS, k = e / RES_COLS;
     xs[c][k] = (j0 + c < n && k < nrhs) ? x[j0 + c + k * ldx] : 0.0;
   }
   __syncthreads();
   if (i >= n) return;
-  double s[RES_RHS];
+  const int jn = (int)(n - j0 < RES_COLS ? n - j0 : RES_COLS);
+  const float* ai = a + i + j0 * lda;
+  if constexpr (NR == 1) {
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    int j = 0;
+    for (; j + 8 <= jn; j += 8) {
+      float av[8];
 #pragma unroll
-  for (int k = 0; k < RES_RHS; ++k) s[k] = 0.0;
-  if (nrhs == 1) {
-    for (int64_t j = j0; j < j1; ++j) s[0] = __builtin_fma((double)a[i + j * lda], xs[j - j0][0], s[0]);
-  } else {
-    for (int64_t j = j0; j < j1; ++j) {
-      double av = (double)a[i + j * lda];
+      for (int q = 0; q < 8; ++q) av[q] = ai[(int64_t)(j + q) * lda];
 #pragma unroll
-      for (int k = 0; k < RES_RHS; ++k) s[k] = __builtin_fma(av, xs[j - j0][k], s[k]);
+      for (int q = 0; q < 8; ++q) s[q & 3] = __builtin_fma((double)av[q], xs[j + q][0], s[q & 3]);
     }
+    for (; j < jn; ++j) s[j & 3] = __builtin_fma((double)ai[(int64_t)j * lda], xs[j][0], s[j & 3]);
+    part[(int64_t)blockIdx.y * n + i] = (s[0] + s[1]) + (s[2] + s[3]);
+  } else {
+    double s[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) s[k] = 0.0;
+    int j = 0;
+    for (; j + 4 <= jn; j += 4) {
+      float av[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) av[q] = ai[(int64_t)(j + q) * lda];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int k = 0; k < NR; ++k) s[k] = __builtin_fma((double)av[q], xs[j + q][k], s[k]);
+    }
+    for (; j < jn; ++j) {
+      const double av = (double)ai[(int64_t)j * lda];
+#pragma unroll
+      for (int k = 0; k < NR; ++k) s[k] = __builtin_fma(av, xs[j][k], s[k]);
+    }
+    for (int k = 0; k < nrhs; ++k) part[((int64_t)blockIdx.y * nrhs + k) * n + i] = s[k];
   }
-  for (int k = 0; k < nrhs; ++k) part[((int64_t)blockIdx.y * nrhs + k) * n + i] = s[k];
 }
 
 __global__ void __launch_bounds__(256) resid_reduce_kernel(int64_t n, int nrhs, int nchunk,
@@ -447,7 +472,11 @@ int residual_launch(remifa_handle_t h, int64_t n, int64_t nrhs, const float* a, 
     int nr = (int)std::min<int64_t>(RES_RHS, nrhs - k0);
     int nchunk = (int)ceil_div(n, RES_COLS);
     dim3 g1((unsigned)ceil_div(n, RES_ROWS), (unsigned)nchunk);
-    resid_partial_kernel<<<g1, 256, 0, h->stream>>>(n, nr, a, lda, x + k0 * ldx, ldx, part);
+    if (nr == 1)
+      resid_partial_kernel<1><<<g1, 256, 0, h->stream>>>(n, nr, a, lda, x + k0 * ldx, ldx, part);
+    else
+      resid_partial_kernel<RES_RHS><<<g1, 256, 0, h->stream>>>(n, nr, a, lda, x + k0 * ldx, ldx,
+                                                               part);
     dim3 g2((unsigned)ceil_div(n, 256), (unsigned)nr);
     resid_reduce_kernel<<<g2, 256, 0, h->stream>>>(n, nr, nchunk, part, b + k0 * ldb, ldb,
                                                    r + k0 * ldr, ldr);

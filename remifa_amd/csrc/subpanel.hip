@@ -11,14 +11,14 @@
 //     diagonal tile, padded with the identity to 64 x 64, stores it back, and
 //     publishes it to a small global table: the factors D and D^T, their operand-rounded
 //     copies (the cross-block GEMM operands) and the pivot reciprocals.
-//  2. vec_kernel (one wavefront per workgroup): every lane owns one row of L
+//  2. vec_kernel (two wavefronts per workgroup): every lane owns one row of L
 //     below the tile or one column of U right of it and solves it against
 //     the tile. A row x of L needs x <- x U^-1 and a column v of U needs
 //     v <- L^-1 v; with M = D for rows and M = D^T for columns (unit
 //     reciprocals for columns) both are the same recurrence over the 64
-//     positions of the lane's vector. The table is read with wave-uniform
-//     addresses (scalar loads), the lane's vector goes through registers in
-//     groups of 8 and is kept in lane-private LDS rows.
+//     positions of the lane's vector. The workgroup copies its half of the
+//     table to LDS (read back as broadcasts); the lane's vector goes through
+//     registers in groups of 8 and is kept in a lane-private LDS row.
 //
 // Inside the slice the work follows the reference's ib-granular semantics:
 //  * within an ib block (the reference's panel kernel, src/lu_nopiv.cu:123-
@@ -270,39 +270,106 @@ __global__ void __launch_bounds__(512) tile_kernel(int sw, int ib, void* __restr
 
 // ---------------------------------------------------------------------------
 // 2. rows of L and columns of U
+constexpr int VL = 128;  // lanes (vectors) per workgroup
+
+template <typename T>
+__device__ __forceinline__ void ld16(const T* p, float* v);  // 16 bytes -> floats
+template <>
+__device__ __forceinline__ void ld16<float>(const float* p, float* v) {
+  const float4 q = *(const float4*)p;
+  v[0] = q.x, v[1] = q.y, v[2] = q.z, v[3] = q.w;
+}
+template <>
+__device__ __forceinline__ void ld16<half_t>(const half_t* p, float* v) {
+  typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+  const h8 q = *(const h8*)p;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = (float)q[e];
+}
+template <typename T>
+__device__ __forceinline__ void st16(T* p, const float* v);  // floats -> 16 bytes
+template <>
+__device__ __forceinline__ void st16<float>(float* p, const float* v) {
+  *(float4*)p = make_float4(v[0], v[1], v[2], v[3]);
+}
+template <>
+__device__ __forceinline__ void st16<half_t>(half_t* p, const float* v) {
+  typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+  h8 q;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) q[e] = (_Float16)v[e];
+  *(h8*)p = q;
+}
+
+// vflags: bit 0 = the U columns may use 16-byte accesses (u and ldu
+// aligned), bit 1 = likewise for the fp16 mirror cp
 template <typename T, int MODE, bool RL>
-__global__ void __launch_bounds__(64) vec_kernel(int64_t cm, int sw, int ib, T* __restrict__ l,
+__global__ void __launch_bounds__(VL) vec_kernel(int64_t cm, int sw, int ib, T* __restrict__ l,
                                                  int64_t ldl, T* __restrict__ u, int64_t ldu,
                                                  half_t* __restrict__ cp, int64_t ldcp,
-                                                 const float* __restrict__ tab, int nl_wg) {
+                                                 const float* __restrict__ tab, int nl_wg,
+                                                 int vflags) {
   using Sm = Sem<MODE>;
-  // lane-private vectors (odd stride: lane t, entry j sits in bank (t + j) % 64)
-  constexpr bool OPID = MODE != SM_F32_TC && MODE != SM_F16_BF;  // op() is the identity
-  __shared__ float Xa[64][SW + 1];                // finished values, then current ones
-  __shared__ float Xo[OPID ? 1 : 64][SW + 1];     // op-rounded finished values
+  constexpr int EV = 16 / sizeof(T);  // entries per 16-byte access
+  // the tile of this workgroup's role (M = D for rows of L, D^T for columns
+  // of U; Mo = op(M); Rs = 1/u_kk for rows, ones for columns), and the
+  // lane-private vectors (odd stride: lane t, entry j sits in bank (t+j)%64)
+  __shared__ float Mt[SW][SW + PD];
+  __shared__ float Mot[SW][SW + PD];
+  __shared__ float Rs[SW];
+  __shared__ float Xa[VL][SW + 1];  // finished values, then current ones
   const int t = threadIdx.x;
   const int wg = blockIdx.x;
-  const bool is_l = wg < nl_wg;  // wave-uniform
-  const int64_t idx = sw + (int64_t)(is_l ? wg : wg - nl_wg) * 64 + t;
+  const bool is_l = wg < nl_wg;  // workgroup-uniform
+  {
+    const float* M = tab + (is_l ? T_D : T_DT);
+    const float* Mo = tab + (is_l ? T_DOP : T_DOPT);
+    constexpr int NQ = SW * SW / 4 / VL;  // 16-byte pieces per lane and matrix
+    float4 qm[NQ], qo[NQ];
+#pragma unroll
+    for (int e = 0; e < NQ; ++e) {
+      const int f = t + e * VL;
+      qm[e] = *(const float4*)&M[f * 4];
+      qo[e] = *(const float4*)&Mo[f * 4];
+    }
+#pragma unroll
+    for (int e = 0; e < NQ; ++e) {
+      const int f = t + e * VL;
+      const int r = f / (SW / 4), c = (f % (SW / 4)) * 4;
+      *(float4*)&Mt[r][c] = qm[e];
+      *(float4*)&Mot[r][c] = qo[e];
+    }
+    if (t < SW) Rs[t] = tab[(is_l ? T_R : T_ONE) + t];
+  }
+  __syncthreads();
+  const int64_t idx = sw + (int64_t)(is_l ? wg : wg - nl_wg) * VL + t;
   if (idx >= cm) return;  // no barriers below: lanes are independent
   // entry j of the lane's vector is p[j * st]; its mirror q[j * sq]
   T* const p = is_l ? l + idx : u + idx * ldu;
   const int64_t st = is_l ? ldl : 1;
   half_t* const q = cp ? (is_l ? cp + idx : cp + idx * ldcp) : nullptr;
   const int64_t sq = is_l ? ldcp : 1;
-  const float* __restrict__ M = tab + (is_l ? T_D : T_DT);
-  const float* __restrict__ Mo = tab + (is_l ? T_DOP : T_DOPT);
-  const float* __restrict__ Rr = tab + (is_l ? T_R : T_ONE);
-  float* const xa = &Xa[t][0];
-  float* const xo = OPID ? xa : &Xo[t][0];
+  const bool vp = !is_l && (vflags & 1);   // 16-byte accesses along the vector
+  const bool vq = !is_l && (vflags & 2);
+  float* const xa = &Xa[t][0];  // (operands of the block sums: op(xa[j]))
 
   // the whole vector in one go (all loads in flight together, addresses
   // clamped in bounds); xa[j] is overwritten by the finished value later
+  {
+    // all loads first (one memory latency), then the LDS row
+    float v[SW];
+    if (vp && sw == SW) {
 #pragma unroll
-  for (int j = 0; j < SW; ++j) {
-    const int jc = j < sw ? j : sw - 1;
-    const float v = (float)p[jc * st];
-    xa[j] = j < sw ? v : 0.0f;
+      for (int j = 0; j < SW; j += EV) ld16<T>(p + j, v + j);
+    } else {
+      T raw[SW];
+#pragma unroll
+      for (int j = 0; j < SW; ++j) raw[j] = p[(j < sw ? j : sw - 1) * st];
+#pragma unroll
+      for (int j = 0; j < SW; ++j) v[j] = j < sw ? (float)raw[j] : 0.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < SW; ++j) xa[j] = v[j];
   }
   for (int c0 = 0; c0 < sw; c0 += G) {
     const int kb = c0 - c0 % ib;  // start of the ib block holding c0
@@ -316,8 +383,8 @@ __global__ void __launch_bounds__(64) vec_kernel(int64_t cm, int sw, int ib, T* 
       for (int c = 0; c < G; ++c) acc[c] = 0.0f;
 #pragma unroll 4
       for (int j = 0; j < kb; ++j) {
-        const float xj = xo[j];
-        const float* mr = Mo + j * SW + c0;
+        const float xj = Sm::op(xa[j]);
+        const float* mr = &Mot[j][c0];
 #pragma unroll
         for (int c = 0; c < G; ++c) acc[c] = Sm::acc_add(acc[c], xj, mr[c]);
       }
@@ -325,31 +392,41 @@ __global__ void __launch_bounds__(64) vec_kernel(int64_t cm, int sw, int ib, T* 
       for (int c = 0; c < G; ++c) xb[c] = Sm::apply(xb[c], acc[c]);
     }
     // pivots of this ib block already finished (earlier groups), in order
-#pragma unroll 2
+#pragma unroll 4
     for (int k = kb; k < c0; ++k) {
       const float lk = xa[k];
-      const float* mr = M + k * SW + c0;
+      const float* mr = &Mt[k][c0];
 #pragma unroll
       for (int c = 0; c < G; ++c) xb[c] = Sm::fnma(lk, mr[c], xb[c]);
     }
     // pivots of this group
 #pragma unroll
     for (int kk = 0; kk < G; ++kk) {
-      const float lk = Sm::mul(xb[kk], Rr[c0 + kk]);
+      const float lk = Sm::mul(xb[kk], Rs[c0 + kk]);
       xb[kk] = lk;
-      const float* mr = M + (c0 + kk) * SW + c0;
+      const float* mr = &Mt[c0 + kk][c0];
 #pragma unroll
       for (int c = kk + 1; c < G; ++c) xb[c] = Sm::fnma(lk, mr[c], xb[c]);
     }
+    if (vp && c0 + G <= sw) {
 #pragma unroll
-    for (int c = 0; c < G; ++c) {
-      if (c0 + c < sw) {
-        p[(c0 + c) * st] = (T)xb[c];
-        if (q) q[(c0 + c) * sq] = (half_t)xb[c];
-      }
-      xa[c0 + c] = xb[c];
-      if (!OPID) xo[c0 + c] = Sm::op(xb[c]);
+      for (int c = 0; c < G; c += EV) st16<T>(p + c0 + c, xb + c);
+    } else {
+#pragma unroll
+      for (int c = 0; c < G; ++c)
+        if (c0 + c < sw) p[(c0 + c) * st] = (T)xb[c];
     }
+    if (q) {
+      if (vq && c0 + G <= sw) {
+        st16<half_t>(q + c0, xb);
+      } else {
+#pragma unroll
+        for (int c = 0; c < G; ++c)
+          if (c0 + c < sw) q[(c0 + c) * sq] = (half_t)xb[c];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < G; ++c) xa[c0 + c] = xb[c];
     const int kend = kb + ib < sw ? kb + ib : sw;
     if (RL && c0 + G >= kend && kend < sw) {
       // the ib block is complete: rank-ib update of the entries right of it
@@ -359,8 +436,8 @@ __global__ void __launch_bounds__(64) vec_kernel(int64_t cm, int sw, int ib, T* 
         for (int c = 0; c < G; ++c) acc[c] = 0.0f;
 #pragma unroll 4
         for (int k = kb; k < kend; ++k) {
-          const float xk = xo[k];
-          const float* mr = Mo + k * SW + c1;
+          const float xk = Sm::op(xa[k]);
+          const float* mr = &Mot[k][c1];
 #pragma unroll
           for (int c = 0; c < G; ++c) acc[c] = Sm::acc_add(acc[c], xk, mr[c]);
         }
@@ -380,10 +457,13 @@ int launch_typed(remifa_handle_t h, int ib, int64_t cm, int sw, void* l, int64_t
   timer_end(h);
   const int64_t rest = cm - sw;
   if (rest <= 0) return 0;
-  const int nl = (int)ceil_div(rest, 64);
+  const int nl = (int)ceil_div(rest, VL);
+  // 16-byte accesses along the U columns need aligned column starts
+  const int vflags = ((uintptr_t)u % 16 == 0 && (ldu * sizeof(T)) % 16 == 0 ? 1 : 0) |
+                     (cp && (uintptr_t)cp % 16 == 0 && (ldcp * 2) % 16 == 0 ? 2 : 0);
   timer_begin(h, KC_PANEL, bytes, 1);
-  vec_kernel<T, MODE, RL><<<2 * nl, 64, 0, h->stream>>>(cm, sw, ib, (T*)l, ldl, (T*)u, ldu, cp,
-                                                         ldcp, tab, nl);
+  vec_kernel<T, MODE, RL><<<2 * nl, VL, 0, h->stream>>>(cm, sw, ib, (T*)l, ldl, (T*)u, ldu, cp,
+                                                         ldcp, tab, nl, vflags);
   RF_HIP(hipGetLastError());
   timer_end(h);
   return 0;

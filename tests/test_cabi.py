@@ -41,3 +41,30 @@ def test_python_package_fails_loudly_without_library(tmp_path, monkeypatch):
     import pytest
     with pytest.raises(ImportError):
         L.lib()
+
+
+def test_cpp_layer_compiles_and_links(tmp_path):
+    """include/remifa.hpp (the reference's names over the C ABI) compiles as
+    host C++ and links against the library (nothing is run)."""
+    import shutil
+    import subprocess
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("no hipcc")
+    src = tmp_path / "caller.cpp"
+    src.write_text(
+        '#include "remifa.hpp"\n'
+        "int main() {\n"
+        "  remifa::Handle h;\n"
+        "  float* a = nullptr; int* info = nullptr; __half* a16 = nullptr;\n"
+        "  remifa::lu_nopiv_ll_f32(h, 0, 0, 256, a, 1, info);\n"
+        "  remifa::lu_nopiv_ll_f32_tc32xtc32(h, 0, 0, 256, a, 1, info, a16, 1);\n"
+        "  remifa::lu_nopiv_ll_f16_tc32xtc32_fp32xfp32_fp32xfp32(h, 0, 0, 256, a16, 1, info);\n"
+        "  return 0;\n"
+        "}\n")
+    libdir = os.path.join(ROOT, "remifa_amd")
+    exe = tmp_path / "caller"
+    r = subprocess.run([hipcc, "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src),
+                        "-L", libdir, "-lremifa_hip", "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

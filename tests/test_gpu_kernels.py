@@ -198,6 +198,33 @@ def test_gemm_mfma_within_bound(handle, gpu, ctype, out, m, n, k):
     assert np.all(np.abs(got.astype(np.float64) - ref) <= bound + 1e-30)
 
 
+@pytest.mark.parametrize("tile", range(8))
+@pytest.mark.parametrize("ctype", ["TC32", "BF16_TC32"])
+@pytest.mark.parametrize("m,n,k", [(300, 257, 513), (512, 384, 256)])
+def test_gemm_every_tile_within_bound(handle, gpu, tile, ctype, m, n, k):
+    """each matrix-core tile shape (forced), aligned and ragged operands"""
+    import torch
+    rf = _rf()
+    rng = np.random.default_rng(tile + m)
+    a, b, c = _gemm_inputs(rng, m, n, k, True)
+    oc = {"TC32": orc.TC32, "BF16_TC32": orc.BF16TC32}[ctype]
+    da, db = to_dev(orc.f32_to_f16_bits(a), gpu), to_dev(orc.f32_to_f16_bits(b), gpu)
+    dc = to_dev(c, gpu)
+    handle.set_gemm_tile(tile)
+    try:
+        rf.gemmex(handle, getattr(rf, ctype), da, db, dc, alpha=-1.0, beta=1.0)
+        torch.cuda.synchronize()
+    finally:
+        handle.set_gemm_tile(-1)
+    got = to_np(dc)
+    ref = orc.gemm_sub(oc, a, b, c, orc.F32)
+    opp = orc.F16 if ctype == "TC32" else orc.BF16
+    aa, bb = orc.round_to(opp, a).astype(np.float64), orc.round_to(opp, b).astype(np.float64)
+    gam = (k + 1) * 2.0**-24 / (1 - (k + 1) * 2.0**-24)
+    bound = 2 * gam * (np.abs(aa) @ np.abs(bb) + np.abs(c))
+    assert np.all(np.abs(got.astype(np.float64) - ref) <= bound + 1e-30)
+
+
 @pytest.mark.parametrize("ctype", ["FP16", "TC16"])
 @pytest.mark.parametrize("m,n,k", [(1, 1, 1), (37, 45, 19), (130, 70, 200), (64, 64, 257)])
 def test_gemm_h16_bitexact(handle, gpu, ctype, m, n, k):
