@@ -213,6 +213,15 @@ int remifa_ir_solve(remifa_handle_t h, int lu_t, int64_t n, int64_t nrhs, const 
                     double *d_x, int64_t ldx, double tol, int maxit, int *iters, double *bwd_err,
                     double *history);
 
+/* The same refinement for the transposed system A^T X = B (transposed
+ * solves with the stored factors, residual B - A^T X in FP64, backward error
+ * with ||A^T||_1). No reference counterpart: it is the adjoint solve of the
+ * differentiable X = A^-1 B (gB = A^-T gX). */
+int remifa_ir_solve_trans(remifa_handle_t h, int lu_t, int64_t n, int64_t nrhs,
+                          const float *d_a, int64_t lda, const void *d_lu, int64_t ldlu,
+                          const double *d_b, int64_t ldb, double *d_x, int64_t ldx, double tol,
+                          int maxit, int *iters, double *bwd_err, double *history);
+
 /* ------------------------------------------------------------ generators */
 /* The benchmark generator on the GPU: a_ij uniform(-1,1) drawn row by row
  * from libstdc++'s minstd_rand0 seeded with seed (tests/genmat.hxx:68-102

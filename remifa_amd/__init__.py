@@ -35,4 +35,6 @@ from .api import (  # noqa: F401  (the reference's entry-point names)
     lu_nopiv_ll_f16_tc32xtc32_fp32xfp32_fp32xfp32,
 )
 
+from . import autograd  # noqa: F401,E402  (torch.autograd.Function X = A^-1 B)
+
 __all__ = [n for n in dir() if not n.startswith("_")]

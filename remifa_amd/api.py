@@ -252,16 +252,18 @@ def residual_f64(handle, a: torch.Tensor, x: torch.Tensor, b: torch.Tensor, r: t
 
 
 def ir_solve(handle, a: torch.Tensor, lu: torch.Tensor, b: torch.Tensor, x: torch.Tensor,
-             tol: float = 1e-14, maxit: int = 30):
-    """Iterative refinement (FP64 residual). Returns (iters, backward_error,
-    history); iters is -1 if tol was not reached."""
+             tol: float = 1e-14, maxit: int = 30, trans: bool = False):
+    """Iterative refinement (FP64 residual) of A X = B, or A^T X = B when
+    trans. Returns (iters, backward_error, history); iters is -1 if tol was
+    not reached."""
     n = a.shape[0]
     nrhs = b.shape[1] if b.dim() == 2 else 1
     it, be = C.c_int(), C.c_double()
     hist = (C.c_double * (maxit + 1))()
-    check(lib().remifa_ir_solve(_h(handle), dtype_code(lu), n, nrhs, _p(a), _ld(a), _p(lu),
-                                _ld(lu), _p(b), _ld(b), _p(x), _ld(x), tol, maxit, C.byref(it),
-                                C.byref(be), C.cast(hist, C.c_void_p)), "remifa_ir_solve")
+    fn = "remifa_ir_solve_trans" if trans else "remifa_ir_solve"
+    check(getattr(lib(), fn)(_h(handle), dtype_code(lu), n, nrhs, _p(a), _ld(a), _p(lu),
+                             _ld(lu), _p(b), _ld(b), _p(x), _ld(x), tol, maxit, C.byref(it),
+                             C.byref(be), C.cast(hist, C.c_void_p)), fn)
     k = it.value if it.value >= 0 else maxit
     return it.value, be.value, list(hist)[: k + 1]
 

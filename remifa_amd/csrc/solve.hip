@@ -293,6 +293,53 @@ __global__ void __launch_bounds__(256) resid_partial_kernel(int64_t n, int nrhs,
   }
 }
 
+// part[chunk][rhs][n] for R = B - A^T X: entry i is column i of A dotted with
+// x over the chunk's rows. 16 lanes per column (64-byte contiguous reads of
+// the column), 16 columns per workgroup; the 16 lane sums of a column are
+// added in lane order (deterministic).
+template <int NR>
+__global__ void __launch_bounds__(256) resid_t_partial_kernel(int64_t n, int nrhs,
+                                                              const float* __restrict__ a,
+                                                              int64_t lda,
+                                                              const double* __restrict__ x,
+                                                              int64_t ldx,
+                                                              double* __restrict__ part) {
+  constexpr int ROWS = NR == 1 ? 2048 : 256;  // rows per chunk
+  __shared__ double xs[ROWS][NR];
+  __shared__ double red[16][16][NR];
+  const int t = threadIdx.x;
+  const int c = t / 16, q = t % 16;
+  const int64_t i = (int64_t)blockIdx.x * 16 + c;  // column of A = entry of r
+  const int64_t j0 = (int64_t)blockIdx.y * ROWS;
+  for (int e = t; e < ROWS * NR; e += 256) {
+    const int jj = e % ROWS, k = e / ROWS;
+    xs[jj][k] = (j0 + jj < n && k < nrhs) ? x[j0 + jj + k * ldx] : 0.0;
+  }
+  __syncthreads();
+  double s[NR];
+#pragma unroll
+  for (int k = 0; k < NR; ++k) s[k] = 0.0;
+  if (i < n) {
+    const int jn = (int)(n - j0 < ROWS ? n - j0 : ROWS);
+    const float* ac = a + j0 + i * lda;
+    for (int jj = q; jj < jn; jj += 16) {
+      const double av = (double)ac[jj];
+#pragma unroll
+      for (int k = 0; k < NR; ++k) s[k] = __builtin_fma(av, xs[jj][k], s[k]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NR; ++k) red[c][q][k] = s[k];
+  __syncthreads();
+  if (q == 0 && i < n) {
+    for (int k = 0; k < nrhs; ++k) {
+      double v = 0.0;
+      for (int r = 0; r < 16; ++r) v += red[c][r][k];
+      part[((int64_t)blockIdx.y * nrhs + k) * n + i] = v;
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256) resid_reduce_kernel(int64_t n, int nrhs, int nchunk,
                                                            const double* __restrict__ part,
                                                            const double* __restrict__ b,
@@ -336,6 +383,16 @@ __global__ void colsum_kernel(int64_t n, const float* __restrict__ a, int64_t ld
     __syncthreads();
   }
   if (threadIdx.x == 0) cs[j] = red[0];
+}
+
+// rs[i] = sum_j |a_ij| (one-norm of A^T), one lane per row
+__global__ void rowsum_kernel(int64_t n, const float* __restrict__ a, int64_t lda,
+                              double* __restrict__ rs) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int64_t j = 0; j < n; ++j) s += fabs((double)a[i + j * lda]);
+  rs[i] = s;
 }
 
 // out[k] = max_i |v_ik| (NaN propagates); one workgroup per column
@@ -459,24 +516,43 @@ int getrs_launch(remifa_handle_t h, int lu_t, int trans, int64_t n, int64_t nrhs
   return st;
 }
 
+// chunks of the residual reduction (rows of A for R = B - A X, rows of A
+// read down the columns for R = B - A^T X)
+static int64_t resid_chunks(int64_t n, int trans, int nr) {
+  return trans ? ceil_div(n, nr == 1 ? 2048 : 256) : ceil_div(n, RES_COLS);
+}
+
 static size_t resid_part_bytes(int64_t n, int64_t nrhs) {
-  return sizeof(double) * (size_t)ceil_div(n, RES_COLS) * (size_t)nrhs * (size_t)n;
+  // the largest of the two layouts
+  const int64_t ch = std::max(resid_chunks(n, 0, (int)nrhs), resid_chunks(n, 1, (int)nrhs));
+  return sizeof(double) * (size_t)ch * (size_t)nrhs * (size_t)n;
 }
 
 int residual_launch(remifa_handle_t h, int64_t n, int64_t nrhs, const float* a, int64_t lda,
                     const double* x, int64_t ldx, const double* b, int64_t ldb, double* r,
-                    int64_t ldr, double* part) {
+                    int64_t ldr, double* part, int trans = 0) {
   if (n == 0 || nrhs == 0) return 0;
-  timer_begin(h, KC_RESID, 2.0 * (double)n * (double)n * (double)nrhs);
+  // bytes: A once per 16 right-hand sides
+  timer_begin(h, KC_RESID, 4.0 * (double)n * (double)n * (double)ceil_div(nrhs, RES_RHS), trans);
   for (int64_t k0 = 0; k0 < nrhs; k0 += RES_RHS) {
     int nr = (int)std::min<int64_t>(RES_RHS, nrhs - k0);
-    int nchunk = (int)ceil_div(n, RES_COLS);
-    dim3 g1((unsigned)ceil_div(n, RES_ROWS), (unsigned)nchunk);
-    if (nr == 1)
-      resid_partial_kernel<1><<<g1, 256, 0, h->stream>>>(n, nr, a, lda, x + k0 * ldx, ldx, part);
-    else
-      resid_partial_kernel<RES_RHS><<<g1, 256, 0, h->stream>>>(n, nr, a, lda, x + k0 * ldx, ldx,
-                                                               part);
+    int nchunk = (int)resid_chunks(n, trans, nr);
+    if (trans) {
+      dim3 g1((unsigned)ceil_div(n, 16), (unsigned)nchunk);
+      if (nr == 1)
+        resid_t_partial_kernel<1><<<g1, 256, 0, h->stream>>>(n, nr, a, lda, x + k0 * ldx, ldx,
+                                                             part);
+      else
+        resid_t_partial_kernel<RES_RHS><<<g1, 256, 0, h->stream>>>(n, nr, a, lda, x + k0 * ldx,
+                                                                   ldx, part);
+    } else {
+      dim3 g1((unsigned)ceil_div(n, RES_ROWS), (unsigned)nchunk);
+      if (nr == 1)
+        resid_partial_kernel<1><<<g1, 256, 0, h->stream>>>(n, nr, a, lda, x + k0 * ldx, ldx, part);
+      else
+        resid_partial_kernel<RES_RHS><<<g1, 256, 0, h->stream>>>(n, nr, a, lda, x + k0 * ldx, ldx,
+                                                                 part);
+    }
     dim3 g2((unsigned)ceil_div(n, 256), (unsigned)nr);
     resid_reduce_kernel<<<g2, 256, 0, h->stream>>>(n, nr, nchunk, part, b + k0 * ldb, ldb,
                                                    r + k0 * ldr, ldr);
@@ -536,10 +612,10 @@ int remifa_residual_f64(remifa_handle_t h, int64_t n, int64_t nrhs, const float*
   return residual_launch(h, n, nrhs, d_a, lda, d_x, ldx, d_b, ldb, d_r, ldr, (double*)h->ws);
 }
 
-int remifa_ir_solve(remifa_handle_t h, int lu_t, int64_t n, int64_t nrhs, const float* d_a,
-                    int64_t lda, const void* d_lu, int64_t ldlu, const double* d_b, int64_t ldb,
-                    double* d_x, int64_t ldx, double tol, int maxit, int* iters, double* bwd_err,
-                    double* history) {
+static int ir_solve_impl(remifa_handle_t h, int trans, int lu_t, int64_t n, int64_t nrhs,
+                         const float* d_a, int64_t lda, const void* d_lu, int64_t ldlu,
+                         const double* d_b, int64_t ldb, double* d_x, int64_t ldx, double tol,
+                         int maxit, int* iters, double* bwd_err, double* history) {
   if (!h) return -1;
   if (lu_t != REMIFA_R_32F && lu_t != REMIFA_R_16F) return -2;
   if (n < 0) return -3;
@@ -579,14 +655,17 @@ int remifa_ir_solve(remifa_handle_t h, int lu_t, int64_t n, int64_t nrhs, const 
   hipStream_t s = h->stream;
   const dim3 gv((unsigned)ceil_div(n, 256), (unsigned)nrhs);
 
-  // ||A||_1 and ||b||_inf per column (fixed for the whole solve)
-  colsum_kernel<<<(unsigned)n, 256, 0, s>>>(n, d_a, lda, cs);
+  // ||op(A)||_1 and ||b||_inf per column (fixed for the whole solve)
+  if (trans)
+    rowsum_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(n, d_a, lda, cs);
+  else
+    colsum_kernel<<<(unsigned)n, 256, 0, s>>>(n, d_a, lda, cs);
   maxreduce_kernel<<<1, 256, 0, s>>>(n, cs, nrm + 2 * nrhs);
   colmax_kernel<<<(unsigned)nrhs, 256, 0, s>>>(n, d_b, ldb, nrm + nrhs);
-  // x0 = (LU)^-1 b in fp32
+  // x0 = op(LU)^-1 b in fp32
   d2f_kernel<<<gv, 256, 0, s>>>(n, (int)nrhs, d_b, ldb, w, n);
   RF_CHECK(trsm_invert_launch(h, lu_t, n, d_lu, ldlu, tw));
-  RF_CHECK(getrs_launch(h, lu_t, 0, n, nrhs, d_lu, ldlu, w, n, tw));
+  RF_CHECK(getrs_launch(h, lu_t, trans, n, nrhs, d_lu, ldlu, w, n, tw));
   axpy_f2d_kernel<<<gv, 256, 0, s>>>(n, w, n, d_x, ldx, 1);
   RF_HIP(hipGetLastError());
 
@@ -594,7 +673,7 @@ int remifa_ir_solve(remifa_handle_t h, int lu_t, int64_t n, int64_t nrhs, const 
   int it = 0, done = -1;
   double worst = 0.0;
   for (;; ++it) {
-    RF_CHECK(residual_launch(h, n, nrhs, d_a, lda, d_x, ldx, d_b, ldb, r, n, part));
+    RF_CHECK(residual_launch(h, n, nrhs, d_a, lda, d_x, ldx, d_b, ldb, r, n, part, trans));
     colmax_kernel<<<(unsigned)nrhs, 256, 0, s>>>(n, r, n, nrm);  // ||r||_inf
     colmax_kernel<<<(unsigned)nrhs, 256, 0, s>>>(n, d_x, ldx, part);  // ||x||_inf (part is free now)
     RF_HIP(hipMemcpyAsync(hn.data(), nrm, sizeof(double) * (2 * nrhs + 1), hipMemcpyDeviceToHost, s));
@@ -625,13 +704,29 @@ int remifa_ir_solve(remifa_handle_t h, int lu_t, int64_t n, int64_t nrhs, const 
     }
     if (it >= maxit || worst != worst) break;
     d2f_kernel<<<gv, 256, 0, s>>>(n, (int)nrhs, r, n, w, n);
-    RF_CHECK(getrs_launch(h, lu_t, 0, n, nrhs, d_lu, ldlu, w, n, tw));
+    RF_CHECK(getrs_launch(h, lu_t, trans, n, nrhs, d_lu, ldlu, w, n, tw));
     axpy_f2d_kernel<<<gv, 256, 0, s>>>(n, w, n, d_x, ldx, 0);
     RF_HIP(hipGetLastError());
   }
   if (iters) *iters = done;
   if (bwd_err) *bwd_err = worst;
   return 0;
+}
+
+int remifa_ir_solve(remifa_handle_t h, int lu_t, int64_t n, int64_t nrhs, const float* d_a,
+                    int64_t lda, const void* d_lu, int64_t ldlu, const double* d_b, int64_t ldb,
+                    double* d_x, int64_t ldx, double tol, int maxit, int* iters, double* bwd_err,
+                    double* history) {
+  return ir_solve_impl(h, 0, lu_t, n, nrhs, d_a, lda, d_lu, ldlu, d_b, ldb, d_x, ldx, tol, maxit,
+                       iters, bwd_err, history);
+}
+
+int remifa_ir_solve_trans(remifa_handle_t h, int lu_t, int64_t n, int64_t nrhs, const float* d_a,
+                          int64_t lda, const void* d_lu, int64_t ldlu, const double* d_b,
+                          int64_t ldb, double* d_x, int64_t ldx, double tol, int maxit,
+                          int* iters, double* bwd_err, double* history) {
+  return ir_solve_impl(h, 1, lu_t, n, nrhs, d_a, lda, d_lu, ldlu, d_b, ldb, d_x, ldx, tol, maxit,
+                       iters, bwd_err, history);
 }
 
 }  // extern "C"
