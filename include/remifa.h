@@ -95,6 +95,12 @@ int remifa_set_tuning(remifa_handle_t handle, int panel_width, int split_k);
  * 6: 256x128, 7: 128x256 (fp32 operands only take 0, 1, 5). Used by the
  * autotuner. */
 int remifa_set_gemm_tile(remifa_handle_t handle, int tile);
+/* Autotuned GEMM table: count rules of 7 int64 each, {operand kind (0 fp16,
+ * 1 bf16, 2 fp32; -1 any), m_lo, m_hi, n_lo, n_hi, tile (-1 = heuristic),
+ * split-K (0 = heuristic)}; the first rule whose ranges (inclusive) hold the
+ * GEMM's M and N decides. Written by remifa_amd/autotune.py. No reference
+ * counterpart. */
+int remifa_set_gemm_rules(remifa_handle_t handle, int count, const int64_t *rules);
 const char *remifa_last_error(void);
 
 /* Per-kernel timing with hipEvents on the handle's stream. While enabled,

@@ -76,6 +76,7 @@ SIGNATURES = {
                         _pi, _pd, _vp],
     "remifa_ir_solve_trans": [_vp, _i, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _d,
                               _i, _pi, _pd, _vp],
+    "remifa_set_gemm_rules": [_vp, _i, _vp],
     "remifa_gen_diagdom": [_vp, _i64, _u64, _d, _vp, _i64],
     "remifa_gen_uniform": [_vp, _i64, _i64, _u64, _vp, _i64],
     "remifa_lu_nopiv_batched": [_vp, _i, _i64, _vp, _vp, _vp, _vp, _i],

@@ -60,11 +60,22 @@ def _p(t) -> C.c_void_p:
 class Handle:
     """remifa_handle_t bound to a torch CUDA (HIP) stream."""
 
-    def __init__(self, stream: torch.cuda.Stream | None = None):
+    def __init__(self, stream: torch.cuda.Stream | None = None, load_tuning: bool = True):
         self._h = C.c_void_p()
         s = torch.cuda.current_stream() if stream is None else stream
         self.stream = s
         check(lib().remifa_create(C.byref(self._h), C.c_void_p(s.cuda_stream)), "remifa_create")
+        if load_tuning:
+            from .autotune import load_rules
+            rules = load_rules()
+            if rules:
+                self.set_gemm_rules(rules)
+
+    def set_gemm_rules(self, rules):
+        """autotuned GEMM table: list of [kind, m_lo, m_hi, n_lo, n_hi, tile, split]"""
+        flat = (C.c_int64 * (7 * len(rules)))(*[int(v) for r in rules for v in r])
+        check(lib().remifa_set_gemm_rules(self._h, len(rules), C.cast(flat, C.c_void_p)),
+              "remifa_set_gemm_rules")
 
     @property
     def value(self):

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -73,6 +74,9 @@ struct remifa_handle_st {
   int panel_width = 0;
   int split_k = 0;
   int gemm_tile = -1;  // -1: chosen per shape
+  // autotuned GEMM rules, first match wins: {opk (-1 any), m_lo, m_hi, n_lo,
+  // n_hi, tile (-1 heuristic), split (0 heuristic)}, bounds inclusive
+  std::vector<std::array<int64_t, 7>> gemm_rules;
 };
 
 namespace remifa {

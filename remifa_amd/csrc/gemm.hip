@@ -222,8 +222,17 @@ int gemm_launch(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t m, in
                    (ldb % chunk == 0) && (m % chunk == 0) && (g.K % chunk == 0);
   const int bk = 64;  // split granularity: a multiple of every slab depth
 
+  // autotuned rule for this shape (first match), else the heuristics
+  int rule_tile = -1, rule_split = 0;
+  for (const auto& r : h->gemm_rules) {
+    if ((r[0] < 0 || r[0] == opk) && m >= r[1] && m <= r[2] && n >= r[3] && n <= r[4]) {
+      rule_tile = (int)r[5];
+      rule_split = (int)r[6];
+      break;
+    }
+  }
   // split-K when the output tiles cannot fill the chip and K is long
-  int split = h->split_k;
+  int split = h->split_k > 0 ? h->split_k : rule_split;
   if (split <= 0) {
     int64_t tiles = ceil_div(m, 128) * ceil_div(n, 128);
     split = 1;
@@ -252,9 +261,11 @@ int gemm_launch(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t m, in
   }
 
   int tile = choose_tile(opk, m, n);
-  if (h->gemm_tile >= 0 && (opk != OPK_F32 || h->gemm_tile == T256x32 ||
-                            h->gemm_tile == T32x256 || h->gemm_tile == T128x128))
-    tile = h->gemm_tile;  // explicit choice (autotuner / experiments)
+  auto fits = [&](int t) {
+    return t >= 0 && (opk != OPK_F32 || t == T256x32 || t == T32x256 || t == T128x128);
+  };
+  if (fits(rule_tile)) tile = rule_tile;  // autotuned table
+  if (fits(h->gemm_tile)) tile = h->gemm_tile;  // explicit override (autotuner runs)
   timer_begin(h, KC_GEMM, flops, (opk << 8) | ((out_t == REMIFA_R_16F) << 4) | tile);
   int st;
   if (opk == OPK_F32) {

@@ -140,6 +140,10 @@ __global__ void __launch_bounds__(256) gemm_mfma_kernel(GemmArgs g) {
   const S* __restrict__ B = (const S*)g.B;
 
   Chunk ra[NA], rb[NB];
+  // VEC: a chunk out of range was loaded from a clamped address; it is
+  // zeroed when stored to LDS, after the compute step, so that the loads stay
+  // in flight while the matrix cores work
+  bool oka[NA], okb[NB];
 
   auto load_slab = [&](int kt) {
     const int64_t kk0 = kbeg + (int64_t)kt * BK;
@@ -150,9 +154,9 @@ __global__ void __launch_bounds__(256) gemm_mfma_kernel(GemmArgs g) {
       int mq = c % (BM / CH), k = c / (BM / CH);
       int64_t gm = m0 + mq * CH, gk = kk0 + k;
       if constexpr (VEC) {
-        bool ok = gm < g.M && gk < kend;
-        Chunk v = *(const Chunk*)(A + (ok ? gm + gk * g.lda : 0));
-        ra[p] = ok ? v : (Chunk)0;
+        const bool ok = gm < g.M && gk < kend;
+        ra[p] = *(const Chunk*)(A + (ok ? gm + gk * g.lda : 0));
+        oka[p] = ok;
       } else {
         Chunk v;
 #pragma unroll
@@ -174,9 +178,9 @@ __global__ void __launch_bounds__(256) gemm_mfma_kernel(GemmArgs g) {
       int kq = c % (BK / CH), n = c / (BK / CH);
       int64_t gk = kk0 + kq * CH, gn = n0 + n;
       if constexpr (VEC) {
-        bool ok = gn < g.N && gk < kend;
-        Chunk v = *(const Chunk*)(B + (ok ? gk + gn * g.ldb : 0));
-        rb[p] = ok ? v : (Chunk)0;
+        const bool ok = gn < g.N && gk < kend;
+        rb[p] = *(const Chunk*)(B + (ok ? gk + gn * g.ldb : 0));
+        okb[p] = ok;
       } else {
         Chunk v;
 #pragma unroll
@@ -198,14 +202,18 @@ __global__ void __launch_bounds__(256) gemm_mfma_kernel(GemmArgs g) {
       int c = tid + 256 * p;
       if (TA % 256 && c >= TA) break;
       int mq = c % (BM / CH), k = c / (BM / CH);
-      *(Chunk*)&As[buf][k * SA + mq * CH] = TR::cvt(ra[p]);
+      Chunk v = ra[p];
+      if constexpr (VEC) v = oka[p] ? v : (Chunk)0;
+      *(Chunk*)&As[buf][k * SA + mq * CH] = TR::cvt(v);
     }
 #pragma unroll
     for (int p = 0; p < NB; ++p) {
       int c = tid + 256 * p;
       if (TB % 256 && c >= TB) break;
       int kq = c % (BK / CH), n = c / (BK / CH);
-      *(Chunk*)&Bs[buf][n * SB + kq * CH] = TR::cvt(rb[p]);
+      Chunk v = rb[p];
+      if constexpr (VEC) v = okb[p] ? v : (Chunk)0;
+      *(Chunk*)&Bs[buf][n * SB + kq * CH] = TR::cvt(v);
     }
   };
 

@@ -133,6 +133,19 @@ int remifa_set_tuning(remifa_handle_t h, int panel_width, int split_k) {
   return 0;
 }
 
+int remifa_set_gemm_rules(remifa_handle_t h, int count, const int64_t* rules) {
+  if (!h) return -1;
+  if (count < 0 || count > 4096) return -2;
+  if (count > 0 && !rules) return -3;
+  std::vector<std::array<int64_t, 7>> r((size_t)count);
+  for (int i = 0; i < count; ++i) {
+    for (int j = 0; j < 7; ++j) r[i][j] = rules[7 * i + j];
+    if (r[i][5] < -1 || r[i][5] > 7 || r[i][6] < 0 || r[i][6] > 64) return -3;
+  }
+  h->gemm_rules.swap(r);
+  return 0;
+}
+
 int remifa_set_gemm_tile(remifa_handle_t h, int tile) {
   if (!h) return -1;
   if (tile < -1 || tile > 7) return -2;

@@ -1,0 +1,130 @@
+"""Batched no-pivot LU + solve (remifa_lu_nopiv_batched): every matrix of a
+ragged batch against the oracle's restatement of the RL driver with
+nb = ib = 32 and TC32 updates (src/lu_nopiv.hxx:53-208), against the FP64
+LU, order-independence of the persistent queue, the batched solve, and
+zero-pivot reporting."""
+import numpy as np
+import pytest
+
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(sizes, seed, alpha=1.2):
+    mats = [orc.gen_diagdom_alpha(int(n), seed + i, alpha) for i, n in enumerate(sizes)]
+    return mats
+
+
+def _pack(mats, gpu, pad=3):
+    """one device buffer, matrix i column-major at offset off[i] with ld n+pad"""
+    import torch
+    lds = [m.shape[0] + pad for m in mats]
+    sizes = [ld * m.shape[1] for ld, m in zip(lds, mats)]
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    buf = torch.zeros(int(sum(sizes)) + 64, dtype=torch.float32)
+    for m, ld, off in zip(mats, lds, offs):
+        n = m.shape[0]
+        blk = np.zeros((n, ld), np.float32)  # column j at row j of blk
+        blk[:, :n] = m.T
+        buf[off:off + ld * n] = torch.from_numpy(blk.reshape(-1))
+    buf = buf.to(gpu)
+    ptrs = torch.tensor([buf.data_ptr() + 4 * int(o) for o in offs], dtype=torch.int64, device=gpu)
+    n_t = torch.tensor([m.shape[0] for m in mats], dtype=torch.int32, device=gpu)
+    ld_t = torch.tensor(lds, dtype=torch.int32, device=gpu)
+    return buf, ptrs, n_t, ld_t, offs, lds
+
+
+def _unpack(buf, mats, offs, lds):
+    host = buf.cpu().numpy()
+    out = []
+    for m, off, ld in zip(mats, offs, lds):
+        n = m.shape[0]
+        out.append(np.asfortranarray(host[off:off + ld * n].reshape(n, ld)[:, :n].T))
+    return out
+
+
+def _run(handle, gpu, mats, ctype=None):
+    import torch
+    import remifa_amd as rf
+    buf, ptrs, n_t, ld_t, offs, lds = _pack(mats, gpu)
+    info = torch.full((len(mats),), -7, dtype=torch.int32, device=gpu)
+    rf.lu_nopiv_batched(handle, rf.TC32 if ctype is None else ctype, n_t, ld_t, ptrs, info,
+                        int(max(m.shape[0] for m in mats)))
+    torch.cuda.synchronize()
+    return _unpack(buf, mats, offs, lds), info.cpu().numpy()
+
+
+def test_batched_lu_matches_oracle(handle, gpu):
+    rng = np.random.default_rng(1)
+    sizes = list(rng.integers(16, 513, 24)) + [2, 3, 31, 32, 33, 64, 65, 512]
+    mats = _batch(sizes, 100)
+    got, info = _run(handle, gpu, mats)
+    assert np.all(info == 0)
+    c = orc.cfg(W=orc.F32, I=orc.TC32, O=orc.TC32, ib=32)
+    for a, lu in zip(mats, got):
+        o = orc.lu_nopiv("rl", c, a, 32)
+        f64 = orc.lu_f64(a)
+        e_o = np.max(np.abs(o - f64)) / np.max(np.abs(f64))
+        d = np.max(np.abs(lu - o)) / np.max(np.abs(o))
+        # same rounding points as the oracle; only the order of the fp32
+        # sums inside the matrix-core update differs
+        assert d <= 8 * max(e_o, 2.0**-24), (a.shape, d, e_o)
+
+
+def test_batched_bf16_update(handle, gpu):
+    import remifa_amd as rf
+    mats = _batch([40, 200, 333], 7)
+    got, info = _run(handle, gpu, mats, ctype=rf.BF16_TC32)
+    assert np.all(info == 0)
+    for a, lu in zip(mats, got):
+        f64 = orc.lu_f64(a)
+        err = np.max(np.abs(lu - f64)) / np.max(np.abs(f64))
+        assert err < 64 * 2.0**-8
+
+
+def test_batched_result_independent_of_batch(handle, gpu):
+    """a matrix's factors do not depend on its neighbours or position"""
+    mats = _batch([100, 300, 17, 512, 64], 55)
+    got1, _ = _run(handle, gpu, mats)
+    order = [3, 0, 4, 2, 1]
+    got2, _ = _run(handle, gpu, [mats[i] for i in order] + _batch([250, 90], 9))
+    for j, i in enumerate(order):
+        assert np.array_equal(got1[i].view(np.uint32), got2[j].view(np.uint32))
+
+
+def test_batched_solve(handle, gpu):
+    import torch
+    import remifa_amd as rf
+    rng = np.random.default_rng(3)
+    sizes = list(rng.integers(16, 513, 12)) + [2, 33]
+    mats = _batch(sizes, 300)
+    buf, ptrs, n_t, ld_t, offs, lds = _pack(mats, gpu)
+    info = torch.zeros(len(mats), dtype=torch.int32, device=gpu)
+    rf.lu_nopiv_batched(handle, rf.TC32, n_t, ld_t, ptrs, info, 512)
+    bs = [rng.uniform(-1, 1, m.shape[0]).astype(np.float32) for m in mats]
+    bbuf = [torch.from_numpy(b).to(gpu) for b in bs]
+    bptrs = torch.tensor([t.data_ptr() for t in bbuf], dtype=torch.int64, device=gpu)
+    rf.getrs_nopiv_batched(handle, n_t, ld_t, ptrs, bptrs, 512)
+    torch.cuda.synchronize()
+    facs = _unpack(buf, mats, offs, lds)
+    c = orc.cfg(W=orc.F32, I=orc.TC32, O=orc.TC32, ib=32)
+    for a, lu, b, xb in zip(mats, facs, bs, bbuf):
+        x = xb.cpu().numpy()
+        want = orc.getrs_f32(lu, b)[:, 0]
+        assert np.max(np.abs(x - want)) <= 1e-5 * max(np.max(np.abs(want)), 1.0)
+        # the factors carry the fp16 operand rounding of the updates: the
+        # solve is as good as the oracle's factors allow
+        xo = orc.getrs_f32(orc.lu_nopiv("rl", c, a, 32), b)[:, 0]
+        be_o = orc.backward_error(a, b.astype(np.float64), xo.astype(np.float64))
+        be = orc.backward_error(a, b.astype(np.float64), x.astype(np.float64))
+        assert be <= 4 * be_o + 1e-7
+
+
+def test_batched_zero_pivot_reported(handle, gpu):
+    a = orc.gen_diagdom_alpha(70, 4, 1.2)
+    a[40, :] = 0.0  # row 40 zero -> u_40,40 = 0 at column 41
+    one = np.zeros((1, 1), np.float32)  # 1 x 1 zero matrix: pivot 1
+    mats = [orc.gen_diagdom_alpha(50, 5, 1.2), a, one]
+    _, info = _run(handle, gpu, mats)
+    assert info[0] == 0 and info[1] == 41 and info[2] == 1
