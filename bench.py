@@ -328,6 +328,29 @@ def main():
     # one more step without per-kernel events, to show their overhead
     t_fac_plain, t_ir_plain, _, _ = step16(False)
 
+    # HBM traffic of the dominant kernel from the committed --pmc summary of
+    # this workload (FETCH_SIZE + WRITE_SIZE per dispatch, calibrated)
+    traffic, traffic_src = None, None
+    bm, bn = rf.api._GEMM_TILE.get(dom & 15, (0, 0, 0, 0))[:2]
+    prof = os.path.join(ROOT, "profiles", f"gemm_mfma_{bm}x{bn}_lu_n{n}_pmc.md")
+    if os.path.exists(prof):
+        import re
+        m = re.search(r"traffic_bytes_per_launch = (\d+)", open(prof).read())
+        if m:
+            traffic, traffic_src = int(m.group(1)), os.path.relpath(prof, ROOT)
+
+    # algorithmic bytes of the wide updates (the 128x128 launches of LL): A
+    # and B in fp16 once, the fp32 panel buffer C read and written once
+    algo_bytes = None
+    if algo == rf.LL:
+        tot, cnt = 0, 0
+        for k in range(1, (n + nb - 1) // nb):
+            ofs = k * nb
+            for (M, N, K) in ((n - ofs, min(nb, n - ofs), ofs), (min(nb, n - ofs), n - ofs, ofs)):
+                tot += 2 * (M * K + K * N) + 8 * M * N
+                cnt += 1
+        algo_bytes = round(tot / cnt) if cnt else None
+
     flops = 2.0 * n**3 / 3.0
     gemm_tflops = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
     dom_tflops = dom_flops / (dom_ms * 1e-3) / 1e12 if dom_ms > 0 else 0.0
@@ -369,7 +392,10 @@ def main():
         },
         "roofline": {"bound": "mfma", "achieved": round(dom_tflops, 2),
                      "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(dom_tflops / PEAK_FP16_TFLOPS, 4), "traffic": None,
+                     "frac": round(dom_tflops / PEAK_FP16_TFLOPS, 4), "traffic": traffic,
+                     "traffic_unit": "bytes per launch (HBM read + write, rocprofv3 FETCH_SIZE/WRITE_SIZE)",
+                     "traffic_source": traffic_src,
+                     "algorithmic_bytes_per_launch": algo_bytes,
                      "kernel": rf.api.gemm_variant_name(dom),
                      "launches_per_step": dom_n // a.steps,
                      "avg_launch_us": round(dom_ms / dom_n * 1e3, 2),

@@ -107,7 +107,9 @@ __global__ void __launch_bounds__(256) gemm_mfma_kernel(GemmArgs g) {
   constexpr int CH = TR::CH;
   constexpr int WTM = BM / WM, WTN = BN / WN;
   constexpr int FM = WTM / 32, FN = WTN / 32;
-  constexpr int PADA = (OPK == OPK_F32) ? 32 : 8;
+  // A rows 64 B apart modulo the 256-byte bank window (16-bit): the four rows
+  // and two m-halves of a transpose read hit all 64 banks once
+  constexpr int PADA = 32;
   constexpr int PADB = (OPK == OPK_F32) ? 4 : 8;
   constexpr int SA = BM + PADA;  // As[k][m]
   constexpr int SB = BK + PADB;  // Bs[n][k]
@@ -127,8 +129,15 @@ __global__ void __launch_bounds__(256) gemm_mfma_kernel(GemmArgs g) {
   const int l32 = lane & 31, lh = lane >> 5;
 
   const int nt = g.tiles_m * g.tiles_n;
+  // grouped raster: GM tile rows at a time, all their tile columns, so the
+  // tiles sharing an A row block (and a B column block) run together on one
+  // XCD and meet in its L2
   const int tile = xcd_tile(blockIdx.x, nt);
-  const int tm = tile % g.tiles_m, tn = tile / g.tiles_m;
+  constexpr int GM = 8;
+  const int gsz = GM * g.tiles_n;
+  const int grp = tile / gsz, first = grp * GM;
+  const int rows = g.tiles_m - first < GM ? g.tiles_m - first : GM;
+  const int tm = first + (tile % gsz) % rows, tn = (tile % gsz) / rows;
   const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
   const int64_t kbeg = (int64_t)blockIdx.z * g.kchunk;
   const int64_t kend = kbeg + g.kchunk < g.K ? kbeg + g.kchunk : g.K;
