@@ -1,11 +1,11 @@
 // Triangular solves with the stored no-pivot factors, the FP64 residual and
 // iterative refinement.
 //
-// TRSM (forward L, backward U and the transposed pair for A^T): the 64 x 64
+// TRSM (forward L, backward U and the transposed pair for A^T): the 128 x 128
 // diagonal blocks are inverted first, all in parallel (diag_inv_kernel), so
 // the sequential chain of the substitution is one small matrix-vector product
-// per block row instead of 64 dependent steps. Then one launch per triangle:
-// workgroups take 64-row block rows in dependency order from an atomic
+// per block row instead of 128 dependent steps. Then one launch per triangle:
+// workgroups take 128-row block rows in dependency order from an atomic
 // ticket, so every block a workgroup waits for belongs to a workgroup that is
 // already running (no deadlock whatever the residency). Blocks finish in
 // order, so one published-block counter (agent-scope release/acquire) tells
@@ -26,7 +26,9 @@
 namespace remifa {
 namespace {
 
-constexpr int NBS = 64;  // block size of the triangular solves
+constexpr int NBS = 128;     // block size of the triangular solves
+constexpr int TT = 4 * NBS;  // threads of a triangular-solve workgroup
+constexpr int QC = NBS / 4;  // columns of a block per lane (column quarter)
 constexpr int RC = 16;   // right-hand sides per launch
 constexpr long long kSpinLimit = 1LL << 28;
 
@@ -42,57 +44,110 @@ __device__ __forceinline__ float tel(const LT* lu, int64_t ld, int64_t r, int64_
 }
 
 // Inverses of the diagonal blocks: blockIdx.y = 0 for the unit lower L_bb,
-// 1 for the upper U_bb. inv[b][r][c] (64 x 64 row-major per block). Lane c
-// computes column c of the inverse by substitution with the block in LDS;
-// rows/columns past n are the identity.
+// 1 for the upper U_bb. inv[b][r][c] (128 x 128 row-major per block); rows
+// and columns past n are the identity. The block is split 2 x 2 into 64 x 64
+// quarters: lanes 0-63 invert the first diagonal quarter and lanes 64-127 the
+// second, lane c computing column c by substitution with the block in LDS;
+// the off-diagonal quarter of the inverse is then -D22^-1 D21 D11^-1 (lower)
+// or -D11^-1 D12 D22^-1 (upper), two 64 x 64 products in LDS.
 template <typename LT>
-__global__ void __launch_bounds__(64) diag_inv_kernel(int64_t n, const LT* __restrict__ lu,
+__global__ void __launch_bounds__(NBS) diag_inv_kernel(int64_t n, const LT* __restrict__ lu,
                                                       int64_t ld, float* __restrict__ inv_l,
                                                       float* __restrict__ inv_u) {
-  __shared__ float D[NBS][NBS + 4];
+  constexpr int H = NBS / 2;
+  __shared__ float D[NBS][NBS + 1];
+  __shared__ float V[NBS][NBS + 1];  // the inverse
   const int t = threadIdx.x;
   const int64_t r0 = (int64_t)blockIdx.x * NBS;
   const bool upper = blockIdx.y == 1;
-  for (int e = t; e < NBS * NBS; e += 64) {
+  for (int e = t; e < NBS * NBS; e += NBS) {
     const int r = e % NBS, c = e / NBS;
     const int64_t R = r0 + r, C = r0 + c;
     D[r][c] = (R < n && C < n) ? ldf(lu + R + C * ld) : (r == c ? 1.0f : 0.0f);
   }
   __syncthreads();
-  float x[NBS];
-  if (!upper) {
+  // diagonal quarters (o is uniform over a wavefront: D reads broadcast)
+  const int o = (t / H) * H, c = t % H;
+  {
+    float x[H];
+    if (!upper) {
 #pragma unroll
-    for (int r = 0; r < NBS; ++r) {
-      float s = (r == t) ? 1.0f : 0.0f;
+      for (int r = 0; r < H; ++r) {
+        float s = (r == c) ? 1.0f : 0.0f;
 #pragma unroll
-      for (int k = 0; k < r; ++k) s = __builtin_fmaf(-D[r][k], x[k], s);
-      x[r] = s;
+        for (int k = 0; k < r; ++k) s = __builtin_fmaf(-D[o + r][o + k], x[k], s);
+        x[r] = s;
+      }
+    } else {
+#pragma unroll
+      for (int r = H - 1; r >= 0; --r) {
+        float s = (r == c) ? 1.0f : 0.0f;
+#pragma unroll
+        for (int k = r + 1; k < H; ++k) s = __builtin_fmaf(-D[o + r][o + k], x[k], s);
+        x[r] = s / D[o + r][o + r];
+      }
     }
-  } else {
 #pragma unroll
-    for (int r = NBS - 1; r >= 0; --r) {
-      float s = (r == t) ? 1.0f : 0.0f;
-#pragma unroll
-      for (int k = r + 1; k < NBS; ++k) s = __builtin_fmaf(-D[r][k], x[k], s);
-      x[r] = s / D[r][r];
+    for (int r = 0; r < H; ++r) {
+      V[o + r][o + c] = x[r];
+      V[H - o + r][o + c] = 0.0f;  // the zero off-diagonal quarter (overwritten below if needed)
     }
   }
-  float* out = (upper ? inv_u : inv_l) + (int64_t)blockIdx.x * NBS * NBS;
+  __syncthreads();
+  // lane (half i0, column j): 32 rows of a 64 x 64 product
+  const int j = t % H, i0 = (t / H) * (H / 2);
+  float acc[H / 2];
+  // step 1: P = D21 V11 (lower) or D12 V22 (upper)
 #pragma unroll
-  for (int r = 0; r < NBS; ++r) out[r * NBS + t] = x[r];
+  for (int i = 0; i < H / 2; ++i) acc[i] = 0.0f;
+  for (int k = 0; k < H; ++k) {
+    const float vk = upper ? V[H + k][H + j] : V[k][j];
+#pragma unroll
+    for (int i = 0; i < H / 2; ++i)
+      acc[i] = __builtin_fmaf(upper ? D[i0 + i][H + k] : D[H + i0 + i][k], vk, acc[i]);
+  }
+  __syncthreads();
+  // P goes to the D quarter it was computed from (no longer needed)
+#pragma unroll
+  for (int i = 0; i < H / 2; ++i) {
+    if (upper)
+      D[i0 + i][H + j] = acc[i];
+    else
+      D[H + i0 + i][j] = acc[i];
+  }
+  __syncthreads();
+  // step 2: V21 = -V22 P (lower) or V12 = -V11 P (upper)
+#pragma unroll
+  for (int i = 0; i < H / 2; ++i) acc[i] = 0.0f;
+  for (int k = 0; k < H; ++k) {
+    const float pk = upper ? D[k][H + j] : D[H + k][j];
+#pragma unroll
+    for (int i = 0; i < H / 2; ++i)
+      acc[i] = __builtin_fmaf(upper ? V[i0 + i][k] : V[H + i0 + i][H + k], pk, acc[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < H / 2; ++i) {
+    if (upper)
+      V[i0 + i][H + j] = -acc[i];
+    else
+      V[H + i0 + i][j] = -acc[i];
+  }
+  __syncthreads();
+  float* out = (upper ? inv_u : inv_l) + (int64_t)blockIdx.x * NBS * NBS;
+  for (int r = 0; r < NBS; ++r) out[r * NBS + t] = V[r][t];
 }
 
 // One block row of T X = B per workgroup (see the file comment). Lane
-// (row, cq) owns row `row` of the block and the column quarter cq (16
-// columns) of every 64 x 64 block it multiplies; the four partial sums are
+// (row, cq) owns row `row` of the block and the column quarter cq (32
+// columns) of every 128 x 128 block it multiplies; the four partial sums are
 // added in LDS in a fixed order. NR: right-hand sides per launch (1 or 16).
 // sync[0] = ticket, sync[1] = error flag, sync[2] = blocks published.
 template <typename LT, bool LOWER, bool TRANS, int NR>
-__global__ void __launch_bounds__(256) trsm_kernel(int64_t n, int nrhs, const LT* __restrict__ lu,
+__global__ void __launch_bounds__(TT) trsm_kernel(int64_t n, int nrhs, const LT* __restrict__ lu,
                                                    int64_t ld, float* __restrict__ b, int64_t ldb,
                                                    const float* __restrict__ inv,
                                                    int* __restrict__ sync, int nblk) {
-  constexpr int CH = NR == 1 ? 8 : 2;  // solution blocks staged in LDS at a time
+  constexpr int CH = NR == 1 ? 2 : 1;  // solution blocks staged in LDS at a time
   constexpr int NP = NR == 1 ? 1 : NR + 1;
   __shared__ float Xs[CH * NBS][NR];
   __shared__ float Red[4][NBS][NP];
@@ -108,12 +163,12 @@ __global__ void __launch_bounds__(256) trsm_kernel(int64_t n, int nrhs, const LT
 
   // this lane's entries of the inverted diagonal block (transposed for the
   // transposed triangles: (T_ii)^-1 = ((D_ii)^-1)^T)
-  float dv[16];
+  float dv[QC];
   {
     const float* ib = inv + (int64_t)bi * NBS * NBS;
 #pragma unroll
-    for (int c = 0; c < 16; ++c) {
-      const int col = cq * 16 + c;
+    for (int c = 0; c < QC; ++c) {
+      const int col = cq * QC + c;
       dv[c] = TRANS ? ib[col * NBS + row] : ib[row * NBS + col];
     }
   }
@@ -129,14 +184,14 @@ __global__ void __launch_bounds__(256) trsm_kernel(int64_t n, int nrhs, const LT
     // this lane's entries of the chunk's blocks of T: issued before the wait
     // (they do not depend on it), with clamped addresses and no branches so
     // the loads are all in flight together
-    float tv[CH][16];
+    float tv[CH][QC];
 #pragma unroll
     for (int q = 0; q < CH; ++q) {
       const int qq = q < cnt ? q : cnt - 1;
       const int bj = LOWER ? s0 + qq : nblk - 1 - (s0 + qq);
-      const int64_t c0 = (int64_t)bj * NBS + cq * 16;
+      const int64_t c0 = (int64_t)bj * NBS + cq * QC;
 #pragma unroll
-      for (int c = 0; c < 16; ++c) {
+      for (int c = 0; c < QC; ++c) {
         const int64_t C = c0 + c < n ? c0 + c : n - 1;
         const float v = tel<LT, TRANS>(lu, ld, grc, C);
         tv[q][c] = (gr < n && c0 + c < n) ? v : 0.0f;
@@ -166,7 +221,7 @@ __global__ void __launch_bounds__(256) trsm_kernel(int64_t n, int nrhs, const LT
     // launch by other workgroups with agent-scope atomic stores; agent-scope
     // atomic loads read them at the device coherence point (no cache
     // invalidation needed).
-    for (int e = t; e < cnt * NBS * NR; e += 256) {
+    for (int e = t; e < cnt * NBS * NR; e += TT) {
       const int rr = e % (cnt * NBS), k = e / (cnt * NBS);
       const int q = rr / NBS, c = rr % NBS;
       const int bj = LOWER ? s0 + q : nblk - 1 - (s0 + q);
@@ -180,10 +235,10 @@ __global__ void __launch_bounds__(256) trsm_kernel(int64_t n, int nrhs, const LT
     for (int q = 0; q < CH; ++q) {
       if (q < cnt) {
 #pragma unroll
-        for (int c = 0; c < 16; ++c) {
+        for (int c = 0; c < QC; ++c) {
 #pragma unroll
           for (int k = 0; k < NR; ++k)
-            acc[k] = __builtin_fmaf(tv[q][c], Xs[q * NBS + cq * 16 + c][k], acc[k]);
+            acc[k] = __builtin_fmaf(tv[q][c], Xs[q * NBS + cq * QC + c][k], acc[k]);
         }
       }
     }
@@ -205,9 +260,9 @@ __global__ void __launch_bounds__(256) trsm_kernel(int64_t n, int nrhs, const LT
 #pragma unroll
   for (int k = 0; k < NR; ++k) y[k] = 0.0f;
 #pragma unroll
-  for (int c = 0; c < 16; ++c) {
+  for (int c = 0; c < QC; ++c) {
 #pragma unroll
-    for (int k = 0; k < NR; ++k) y[k] = __builtin_fmaf(dv[c], Xs[cq * 16 + c][k], y[k]);
+    for (int k = 0; k < NR; ++k) y[k] = __builtin_fmaf(dv[c], Xs[cq * QC + c][k], y[k]);
   }
 #pragma unroll
   for (int k = 0; k < NR; ++k) Red[cq][row][k] = y[k];
@@ -438,10 +493,10 @@ int launch_trsm(remifa_handle_t h, int64_t n, int nrhs, const void* lu, int64_t 
   fill_int_kernel<<<1, 256, 0, h->stream>>>(sync, 3, 0);
   if (nrhs == 1)
     trsm_kernel<LT, LOWER, TRANS, 1>
-        <<<nblk, 256, 0, h->stream>>>(n, nrhs, (const LT*)lu, ld, b, ldb, inv, sync, nblk);
+        <<<nblk, TT, 0, h->stream>>>(n, nrhs, (const LT*)lu, ld, b, ldb, inv, sync, nblk);
   else
     trsm_kernel<LT, LOWER, TRANS, RC>
-        <<<nblk, 256, 0, h->stream>>>(n, nrhs, (const LT*)lu, ld, b, ldb, inv, sync, nblk);
+        <<<nblk, TT, 0, h->stream>>>(n, nrhs, (const LT*)lu, ld, b, ldb, inv, sync, nblk);
   RF_HIP(hipGetLastError());
   return 0;
 }
@@ -491,10 +546,10 @@ static int trsm_invert_launch(remifa_handle_t h, int lu_t, int64_t n, const void
   const int nblk = (int)ceil_div(n, NBS);
   timer_begin(h, KC_TRSM, (double)n * NBS * (lu_t == REMIFA_R_16F ? 2 : 4) * 2, 1);
   if (lu_t == REMIFA_R_16F)
-    diag_inv_kernel<half_t><<<dim3(nblk, 2), 64, 0, h->stream>>>(n, (const half_t*)lu, ld,
+    diag_inv_kernel<half_t><<<dim3(nblk, 2), NBS, 0, h->stream>>>(n, (const half_t*)lu, ld,
                                                                   w.inv_l, w.inv_u);
   else
-    diag_inv_kernel<float><<<dim3(nblk, 2), 64, 0, h->stream>>>(n, (const float*)lu, ld, w.inv_l,
+    diag_inv_kernel<float><<<dim3(nblk, 2), NBS, 0, h->stream>>>(n, (const float*)lu, ld, w.inv_l,
                                                                  w.inv_u);
   RF_HIP(hipGetLastError());
   timer_end(h);

@@ -98,8 +98,8 @@ struct Sem {
 // ---------------------------------------------------------------------------
 // 1. the diagonal tile, blocked by 8-column panels: eight wavefronts, lane i
 // holds row i, wavefront w owns columns 8w..8w+7 in registers. Stage g:
-// wavefront g factors its panel (8 pivots, broadcasts inside the wavefront
-// only) and shares the multipliers through LDS; after one barrier every
+// wavefront g factors its panel (8 pivots, the pivot row broadcast with
+// readlane inside the wavefront) and shares the multipliers through LDS; after one barrier every
 // wavefront right of it applies the panel's 8 pivots to its own columns in
 // pivot order. Every entry receives the same sequence of fused updates as in
 // a pivot-by-pivot elimination. ib (a multiple of 8) decides where the block
@@ -109,17 +109,14 @@ constexpr int TP = 8;  // panel width = columns per wavefront
 
 struct TileLds {
   float Lp[2][SW][TP + 1];  // multipliers of the current panel (double-buffered)
-  float Bc[SW / TP][TP + 1];// per wavefront: the pivot row of its columns (+ 1/u_kk)
   float Uo[SW][SW + PD];    // op-rounded rows of U (row k published at pivot k)
   float Xo[SW][SW + 1];     // op-rounded multipliers
   float R[SW];              // reciprocals of the pivots
 };
 
-// LDS writes of this wavefront complete and stay ordered before its reads
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+// value of v in lane `lane` (wave-uniform) for every lane of the wavefront
+__device__ __forceinline__ float bcast_lane(float v, int lane) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
 }
 
 template <int MODE, bool RL>
@@ -171,30 +168,28 @@ __global__ void __launch_bounds__(512) tile_kernel(int sw, int ib, void* __restr
       }
     }
     if (w == g) {
-      // factor the panel: pivot by pivot inside this wavefront
+      // factor the panel: pivot by pivot inside this wavefront; lane k's
+      // row is read by every lane with readlane (no LDS round trip)
 #pragma unroll
       for (int kk = 0; kk < TP; ++kk) {
         const int k = p0 + kk;
-        if (i == k) {
+        float u[TP];
 #pragma unroll
-          for (int jj = kk; jj < TP; ++jj) {
-            S.Bc[w][jj] = x[jj];
-            S.Uo[k][cb + jj] = Sm::op(x[jj]);
-          }
-          const float r = Sm::rcp(x[kk]);
-          S.Bc[w][TP] = r;
+        for (int jj = kk; jj < TP; ++jj) u[jj] = bcast_lane(x[jj], k);
+        const float r = Sm::rcp(u[kk]);
+        if (i == k) {
           S.R[k] = r;
+#pragma unroll
+          for (int jj = kk; jj < TP; ++jj) S.Uo[k][cb + jj] = Sm::op(x[jj]);
         }
-        wave_lds_sync();
         if (i > k) {
-          const float lk = Sm::mul(x[kk], S.Bc[w][TP]);
+          const float lk = Sm::mul(x[kk], r);
           x[kk] = lk;
           S.Lp[g & 1][i][kk] = lk;
           S.Xo[i][k] = Sm::op(lk);
 #pragma unroll
-          for (int jj = kk + 1; jj < TP; ++jj) x[jj] = Sm::fnma(lk, S.Bc[w][jj], x[jj]);
+          for (int jj = kk + 1; jj < TP; ++jj) x[jj] = Sm::fnma(lk, u[jj], x[jj]);
         }
-        wave_lds_sync();
       }
     }
     __syncthreads();
@@ -202,23 +197,23 @@ __global__ void __launch_bounds__(512) tile_kernel(int sw, int ib, void* __restr
       // the panel's pivots on this wavefront's columns, in pivot order: for
       // every row inside the block, for the rows below it only within it
       const bool upd = i < kend || cb < kend;
+      float lp[TP];
+#pragma unroll
+      for (int kk = 0; kk < TP; ++kk) lp[kk] = S.Lp[g & 1][i][kk];
 #pragma unroll
       for (int kk = 0; kk < TP; ++kk) {
         const int k = p0 + kk;
+        float u[TP];
+#pragma unroll
+        for (int jj = 0; jj < TP; ++jj) u[jj] = bcast_lane(x[jj], k);
         if (i == k) {
 #pragma unroll
-          for (int jj = 0; jj < TP; ++jj) {
-            S.Bc[w][jj] = x[jj];
-            S.Uo[k][cb + jj] = Sm::op(x[jj]);
-          }
+          for (int jj = 0; jj < TP; ++jj) S.Uo[k][cb + jj] = Sm::op(x[jj]);
         }
-        wave_lds_sync();
         if (i > k && upd) {
-          const float lk = S.Lp[g & 1][i][kk];
 #pragma unroll
-          for (int jj = 0; jj < TP; ++jj) x[jj] = Sm::fnma(lk, S.Bc[w][jj], x[jj]);
+          for (int jj = 0; jj < TP; ++jj) x[jj] = Sm::fnma(lp[kk], u[jj], x[jj]);
         }
-        wave_lds_sync();
       }
       if (RL && p0 + TP == kend && cb >= kend) {
         // block complete: right-looking rank-ib update of the trailing part
