@@ -33,11 +33,16 @@ FAMILIES = {
 }
 
 
-def driver_shapes(n: int, nb: int, pw: int = 64):
-    """(family, M, N, K) of representative GEMMs of the LL driver at order n"""
+def driver_shapes(n: int, nb: int, pw: int = 64, samples: int = 16):
+    """(family, M, N, K) of representative GEMMs of the LL driver at order n:
+    the updates of `samples` outer blocks spread evenly over the
+    factorization (the best split-K and tile change along it: early updates
+    have many output tiles and short K, late ones few tiles and long K)"""
     out = []
-    for frac in (0.25, 0.5, 0.75):
-        ofs = max(nb, int(n * frac) // nb * nb)
+    nblk = (n + nb - 1) // nb
+    step = max(1, (nblk - 1) // samples)
+    for kblk in range(1, nblk, step):
+        ofs = kblk * nb
         m = n - ofs
         out.append(("wide", m, nb, ofs))          # block column (:239-304)
         out.append(("wide", nb, m - nb, ofs))     # block row (:311-410)

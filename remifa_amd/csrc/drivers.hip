@@ -68,7 +68,12 @@ struct Arena {
   }
 };
 
-constexpr size_t kGemmReserve = (size_t)64 << 20;  // split-K partials
+// split-K partials of the update GEMMs: up to 16 slices of the largest
+// block-column update, at most 256 MiB
+size_t gemm_reserve(int64_t m, int64_t nb) {
+  const size_t full = (size_t)16 * 4 * (size_t)m * (size_t)nb;
+  return std::min<size_t>((size_t)256 << 20, std::max<size_t>((size_t)16 << 20, full));
+}
 
 struct Ctx {
   remifa_handle_t h;
@@ -169,7 +174,7 @@ size_t lu_workspace_bytes(int algo, const remifa_lu_config_t& c, int64_t m, int6
     b += ((size_t)m * nb * 4 + 255) & ~(size_t)255;  // block column buffer
     b += ((size_t)nb * m * 4 + 255) & ~(size_t)255;  // block row buffer
   }
-  return b + kGemmReserve;
+  return b + gemm_reserve(m, nb);
 }
 
 int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, int64_t m, int64_t n,

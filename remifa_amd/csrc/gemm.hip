@@ -12,15 +12,22 @@
 //    the next K slab is fetched from HBM while the matrix cores work on the
 //    current one. A slabs are stored m-contiguous ([k][m]) and B slabs
 //    k-contiguous ([n][k]), both with 16-byte stores; B fragments are one
-//    16-byte LDS read, A fragments are gathered element-wise (conflict-free
+//    16-byte LDS read, A fragments two LDS transpose reads (conflict-free
 //    thanks to the row padding).
 //  * Workgroup ids are remapped so that each of the 8 XCDs (which own
 //    private L2s) processes a contiguous range of tiles.
 //  * Optional split-K: slice z of K writes alpha*partial to a workspace and a
-//    reduction kernel adds the slices in order (deterministic).
+//    reduction kernel adds the slices in order (deterministic). Unless a rule
+//    or the caller fixes it, the split count comes from a cost model: whole
+//    rounds of resident workgroups times the K slabs per workgroup, plus the
+//    reduction's traffic. The update GEMMs of an LU have few output tiles
+//    and long K late in the factorization, where one round of workgroups
+//    would leave most CUs idle.
 //
 // fp16-accumulating compute types (FP16, TC16: cublasHgemm / GemmEx 16F) run
 // on a VALU kernel with one correctly rounded fp16 fma per k, in k order.
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace remifa {
@@ -151,6 +158,49 @@ int choose_tile(int opk, int64_t M, int64_t N) {
   return T128x128;
 }
 
+// tile geometry for the split-K cost model: BM, BN and the workgroups a CU
+// holds at once (LDS / register limited)
+struct TileGeom {
+  int bm, bn, slots;
+};
+TileGeom tile_geom(int tile) {
+  switch (tile) {
+    case T256x32: return {256, 32, 1};
+    case T32x256: return {32, 256, 1};
+    case T256x64: return {256, 64, 1};
+    case T64x256: return {64, 256, 1};
+    case T256x256: return {256, 256, 1};
+    case T256x128: return {256, 128, 1};
+    case T128x256: return {128, 256, 1};
+    default: return {128, 128, 2};
+  }
+}
+
+// split count minimising the modelled time: ceil(tiles * s / resident) rounds
+// of ceil(slabs / s) K slabs each, plus the reduction (s partials read, C
+// read and written) and its launch. rate: matrix-core flop rate of one CU
+// running this kernel at full occupancy (measured: ~3.5 TFLOP/s for 16-bit
+// operands on MI355X, ~0.45 for fp32). kmax_split: splits whose partials
+// exceed `avail` bytes are not considered.
+int split_model(int64_t M, int64_t N, int64_t K, TileGeom tg, int num_cu, double rate, int bk,
+                size_t avail) {
+  const int64_t tiles = ceil_div(M, tg.bm) * ceil_div(N, tg.bn);
+  const int64_t res = (int64_t)tg.slots * num_cu;
+  const int64_t nk = ceil_div(K, bk);
+  const double t_slab = 2.0 * tg.bm * tg.bn * bk * tg.slots / rate;
+  int best = 1;
+  double tbest = (double)ceil_div(tiles, res) * nk * t_slab;
+  for (int s = 2; s <= 16 && s <= nk; ++s) {
+    const int64_t kc = ceil_div(nk, s);
+    const int64_t se = ceil_div(nk, kc);  // slices actually used
+    if (sizeof(float) * (size_t)se * (size_t)M * (size_t)N > avail) break;
+    const double t = (double)ceil_div(tiles * se, res) * kc * t_slab +
+                     (double)(se + 2) * M * N * 4.0 / 3.0e12 + 4.0e-6;
+    if (t < tbest * 0.97) best = (int)se, tbest = t;
+  }
+  return best;
+}
+
 template <int OPK, typename OutT>
 int dispatch_tiles(remifa_handle_t h, int tile, GemmArgs g, int split, bool vec) {
   switch (tile) {
@@ -231,13 +281,21 @@ int gemm_launch(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t m, in
       break;
     }
   }
-  // split-K when the output tiles cannot fill the chip and K is long
+  int tile = choose_tile(opk, m, n);
+  auto fits = [&](int t) {
+    return t >= 0 && (opk != OPK_F32 || t == T256x32 || t == T32x256 || t == T128x128);
+  };
+  if (fits(rule_tile)) tile = rule_tile;  // autotuned table
+  if (fits(h->gemm_tile)) tile = h->gemm_tile;  // explicit override (autotuner runs)
+
+  // split-K: the caller's, the rule's, else the cost model's
   int split = h->split_k > 0 ? h->split_k : rule_split;
   if (split <= 0) {
-    int64_t tiles = ceil_div(m, 128) * ceil_div(n, 128);
-    split = 1;
-    while (tiles * split < 2 * (int64_t)h->num_cu && g.K / (split * 2) >= 1024 && split < 16)
-      split *= 2;
+    const bool standalone = h->ws_used == 0 && !h->ws_user;
+    const size_t avail = standalone ? std::max<size_t>(h->ws_bytes, (size_t)256 << 20)
+                                    : (h->ws_bytes > h->ws_used ? h->ws_bytes - h->ws_used : 0);
+    split = split_model(m, n, g.K, tile_geom(tile), h->num_cu,
+                        opk == OPK_F32 ? 0.45e12 : 3.5e12, bk, avail);
   }
   if (g.K == 0) split = 1;
   g.kchunk = ceil_div(ceil_div(g.K > 0 ? g.K : 1, split), bk) * bk;
@@ -260,12 +318,6 @@ int gemm_launch(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t m, in
     g.ldw = m;
   }
 
-  int tile = choose_tile(opk, m, n);
-  auto fits = [&](int t) {
-    return t >= 0 && (opk != OPK_F32 || t == T256x32 || t == T32x256 || t == T128x128);
-  };
-  if (fits(rule_tile)) tile = rule_tile;  // autotuned table
-  if (fits(h->gemm_tile)) tile = h->gemm_tile;  // explicit override (autotuner runs)
   timer_begin(h, KC_GEMM, flops, (opk << 8) | ((out_t == REMIFA_R_16F) << 4) | tile);
   int st;
   if (opk == OPK_F32) {
