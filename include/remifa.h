@@ -188,11 +188,35 @@ int remifa_convert(remifa_handle_t h, int src_t, int dst_t, int64_t m, int64_t n
 
 /* src/gemm.cuh:24-40 gemmex<Impl,Op,AccType,InType,OutType>: C = alpha A B +
  * beta C (no transposes). ctype is a remifa_compute_t, in_t the storage type
- * of A and B (f32 for FP32; f16 for FP16/TC16/TC32/BF16_TC32), out_t the
- * storage type of C (f32 or f16; f16 for FP16/TC16). */
+ * of A and B (f32 for FP32; f16 for FP16/TC16/TC32/BF16_TC32; f16 or f32
+ * for FP8_TC32, quantized per 32-long k block as remifa_quantize_fp8 does,
+ * in the workspace), out_t the storage type of C (f32 or f16; f16 for
+ * FP16/TC16). */
 int remifa_gemmex(remifa_handle_t h, int ctype, int in_t, int out_t, int m, int n, int k,
                   float alpha, const void *a, int lda, const void *b, int ldb, float beta, void *c,
                   int ldc);
+
+/* FP8 update operands (no reference counterpart: the MI355X FP8_TC32 update).
+ * Quantize the m x k block a (column-major, ld lda, f32 or f16) by rows
+ * (rows != 0: one scale per row and 32-long k block, the A operand) or its
+ * k x m transpose view by columns (rows == 0: a is k x m, one scale per
+ * column and 32-long k block, the B operand): codes q[r * ldq + kk] (OCP
+ * e4m3, nearest even, saturating), scale bytes s[r * lds + kk / 32] meaning
+ * 2^(s - 127), the scale being 2^(floor(log2 amax) - 8) of the block (2^0
+ * for an all-zero block). Needs q 16-byte aligned, ldq a multiple of 16 and
+ * >= k rounded up to 32, lds >= ceil(k / 32). */
+int remifa_quantize_fp8(remifa_handle_t h, int in_t, int rows, int64_t m, int64_t k,
+                        const void *a, int64_t lda, uint8_t *q, int64_t ldq, uint8_t *s,
+                        int64_t lds);
+/* C = alpha A B + beta C (C m x n column-major, f32 or f16) from quantized
+ * operands: A rows (qa, sa) and B columns (qb, sb) as produced above, k a
+ * multiple of 32; code buffers 16-byte aligned with ld multiples of 16,
+ * scale buffers 4-byte aligned with ld multiples of 4 and >= ceil(k / 128)
+ * * 4. Products of e4m3 codes on the scaled matrix cores, fp32 sums. */
+int remifa_gemm_fp8(remifa_handle_t h, int out_t, int64_t m, int64_t n, int64_t k, float alpha,
+                    const uint8_t *qa, int64_t ldqa, const uint8_t *sa, int64_t ldsa,
+                    const uint8_t *qb, int64_t ldqb, const uint8_t *sb, int64_t ldsb, float beta,
+                    void *c, int64_t ldc);
 
 /* ------------------------------------------------------------ solves */
 /* Identity-pivot getrs (the reference's check solve host_getrs with

@@ -72,6 +72,7 @@ def _declare(L):
         "orc_convert": (C.c_int, [C.c_int, C.c_int, i64, i64, vp, i64, vp, i64]),
         "orc_lu_nopiv_panel": (C.c_int, [C.c_int, i64, i64, i64, vp, i64, vp, i64,
                                          vp, i64, vp]),
+        "orc_quantise_e4m3_codes": (None, [i64, i64, vp, i64, vp, vp, C.c_int]),
         "orc_gemm_sub": (None, [C.c_int, i64, i64, i64, vp, i64, vp, i64, vp, i64,
                                 C.c_int]),
         "orc_lu_nopiv_ll_typed": (C.c_int, [C.POINTER(LuCfg), i64, i64, i64, vp, i64]),
@@ -184,6 +185,19 @@ def lu_nopiv_panel(prec: int, ib: int, m: int, n: int, d, l, u):
     if st:
         raise ValueError(f"orc_lu_nopiv_panel: bad argument {-st}")
     return l, u, int(stat[0])
+
+
+def quantise_e4m3(a, rows: bool = True):
+    """Codes and scale exponents of the FP8TC32 operand format: a is m x k
+    quantised by rows (rows=True) or k x m by columns (rows=False).
+    Returns codes uint8 [r, k] and exponents int32 [r, ceil(k/32)]
+    (scale = 2^exponent, value = e4m3(code) * scale)."""
+    a = _fortran(a, np.float32)
+    r, k = (a.shape[0], a.shape[1]) if rows else (a.shape[1], a.shape[0])
+    codes = np.zeros((r, max(k, 1)), np.uint8)
+    sexp = np.zeros((r, max((k + 31) // 32, 1)), np.int32)
+    lib().orc_quantise_e4m3_codes(r, k, _p(a), max(a.shape[0], 1), _p(codes), _p(sexp), int(rows))
+    return codes[:, :k], sexp[:, :(k + 31) // 32]
 
 
 def gemm_sub(ctype: int, a, b, c, cprec: int = F32) -> np.ndarray:

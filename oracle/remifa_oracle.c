@@ -502,14 +502,14 @@ static float e4m3_block_scale(float amax)
     return ldexpf(1.0f, e - 1 - 8);
 }
 
-/* quantise operand panels. a_q: m x k, rows scaled per 32-long k block;
- * b_q: k x n, columns scaled per 32-long k block. Blocks start at k = 0
- * of the given panels (callers align k to the factor's column index). */
-static void quantise_e4m3(int64_t m, int64_t k, const float *a, int64_t lda,
-                          float *aq, int rows)
+/* codes and scale exponents of the e4m3 operand format: for row r (rows =
+ * 1: a is m x k, the A operand) or column r (rows = 0: a is k x m, the B
+ * operand) and each 32-long k block b starting at k = 0, sexp[r * nkb + b] =
+ * log2 of the block scale and codes[r * k + kk] = e4m3(a / scale). */
+void orc_quantise_e4m3_codes(int64_t m, int64_t k, const float *a, int64_t lda, uint8_t *codes,
+                             int32_t *sexp, int rows)
 {
-    /* rows = 1: blocks run along a row (a is m x k, scale per row+kblock)
-     * rows = 0: a is k x m (B operand), scale per column+kblock */
+    int64_t nkb = (k + 31) / 32;
     for (int64_t r = 0; r < m; ++r)
         for (int64_t k0 = 0; k0 < k; k0 += 32) {
             int64_t k1 = k0 + 32 < k ? k0 + 32 : k;
@@ -519,12 +519,35 @@ static void quantise_e4m3(int64_t m, int64_t k, const float *a, int64_t lda,
                 amax = fmaxf(amax, fabsf(v));
             }
             float s = e4m3_block_scale(amax);
+            int e;
+            frexpf(s, &e);
+            sexp[r * nkb + k0 / 32] = e - 1;
             for (int64_t kk = k0; kk < k1; ++kk) {
                 float v = rows ? a[r + lda * kk] : a[kk + lda * r];
-                float q = orc_e4m3_to_f32(orc_f32_to_e4m3(v / s)) * s;
-                if (rows) aq[r + m * kk] = q; else aq[kk + k * r] = q;
+                codes[r * k + kk] = orc_f32_to_e4m3(v / s);
             }
         }
+}
+
+/* quantise operand panels. a_q: m x k, rows scaled per 32-long k block;
+ * b_q: k x n, columns scaled per 32-long k block. Blocks start at k = 0
+ * of the given panels (callers align k to the factor's column index). */
+static void quantise_e4m3(int64_t m, int64_t k, const float *a, int64_t lda,
+                          float *aq, int rows)
+{
+    /* rows = 1: blocks run along a row (a is m x k, scale per row+kblock)
+     * rows = 0: a is k x m (B operand), scale per column+kblock */
+    int64_t nkb = (k + 31) / 32;
+    uint8_t *codes = malloc((size_t)(m * k > 0 ? m * k : 1));
+    int32_t *sexp = malloc(sizeof(int32_t) * (size_t)(m * nkb > 0 ? m * nkb : 1));
+    orc_quantise_e4m3_codes(m, k, a, lda, codes, sexp, rows);
+    for (int64_t r = 0; r < m; ++r)
+        for (int64_t kk = 0; kk < k; ++kk) {
+            float q = ldexpf(orc_e4m3_to_f32(codes[r * k + kk]), sexp[r * nkb + kk / 32]);
+            if (rows) aq[r + m * kk] = q; else aq[kk + k * r] = q;
+        }
+    free(codes);
+    free(sexp);
 }
 
 void orc_gemm_sub(int ctype, int64_t m, int64_t n, int64_t k,

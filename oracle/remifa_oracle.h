@@ -61,6 +61,13 @@ int orc_lu_nopiv_panel(int prec, int64_t ib, int64_t m, int64_t n,
                        void *u, int64_t ldu, int *stat);
 
 /* ---- GEMM update semantics: src/gemm.cu (C -= A*B, alpha=-1, beta=1) ---- */
+/* e4m3 operand format of the FP8TC32 update (a MI355X addition, no
+ * reference counterpart): for row r (rows = 1, a is m x k) or column r
+ * (rows = 0, a is k x m) and each 32-long k block b from k = 0,
+ * sexp[r * ceil(k/32) + b] = floor(log2 amax) - 8 (0 for an all-zero block)
+ * and codes[r * k + kk] = e4m3(a / 2^sexp), nearest even, saturating. */
+void orc_quantise_e4m3_codes(int64_t m, int64_t k, const float *a, int64_t lda, uint8_t *codes,
+                             int32_t *sexp, int rows);
 void orc_gemm_sub(int ctype, int64_t m, int64_t n, int64_t k,
                   const float *a, int64_t lda, const float *b, int64_t ldb,
                   float *c, int64_t ldc, int cprec);

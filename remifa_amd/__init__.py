@@ -23,7 +23,8 @@ from ._lib import (  # noqa: F401  (re-exported enums)
 )
 from .api import (  # noqa: F401
     Handle, colmajor_empty, to_colmajor, dtype_code,
-    lu_nopiv, lu_nopiv_panel, convert, gemmex, getrs_nopiv, residual_f64, ir_solve,
+    lu_nopiv, lu_nopiv_panel, convert, gemmex, quantize_fp8, gemm_fp8, getrs_nopiv, residual_f64,
+    ir_solve,
     gen_diagdom, gen_uniform, lu_workspace_size, lu_nopiv_batched, getrs_nopiv_batched,
     REFERENCE_HELPERS,
 )

@@ -70,6 +70,9 @@ SIGNATURES = {
     "remifa_lu_nopiv_panel": [_vp, _i, _i, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp],
     "remifa_convert": [_vp, _i, _i, _i64, _i64, _vp, _i64, _vp, _i64],
     "remifa_gemmex": [_vp, _i, _i, _i, _i, _i, _i, _f, _vp, _i, _vp, _i, _f, _vp, _i],
+    "remifa_quantize_fp8": [_vp, _i, _i, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64],
+    "remifa_gemm_fp8": [_vp, _i, _i64, _i64, _i64, _f, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64,
+                        _f, _vp, _i64],
     "remifa_getrs_nopiv": [_vp, _i, _i, _i64, _i64, _vp, _i64, _vp, _i64],
     "remifa_residual_f64": [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64],
     "remifa_ir_solve": [_vp, _i, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _d, _i,

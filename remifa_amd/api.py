@@ -145,6 +145,9 @@ def gemm_variant_name(v: int) -> str:
     """Template instantiation of gemm_mfma_kernel for a GEMM variant code."""
     opk, out16, tile = v >> 8, (v >> 4) & 15, v & 15
     bm, bn, wm, wn = _GEMM_TILE.get(tile, (0, 0, 0, 0))
+    if opk == 3:
+        return (f"gemm_fp8_kernel<{'_Float16' if out16 else 'float'}> "
+                f"(scaled e4m3 operands, 128x128 tile)")
     return (f"gemm_mfma_kernel<{opk}, {'_Float16' if out16 else 'float'}, {bm}, {bn}, {wm}, {wn}>"
             f" ({_GEMM_OPK.get(opk, '?')} operands, {bm}x{bn} tile)")
 
@@ -244,6 +247,30 @@ def gemmex(handle, ctype: int, a: torch.Tensor, b: torch.Tensor, c: torch.Tensor
     n = b.shape[1]
     check(lib().remifa_gemmex(_h(handle), ctype, dtype_code(a), dtype_code(c), m, n, k, alpha,
                               _p(a), _ld(a), _p(b), _ld(b), beta, _p(c), _ld(c)), "remifa_gemmex")
+
+
+def quantize_fp8(handle, a: torch.Tensor, rows: bool = True):
+    """FP8 operand of a column-major f32/f16 matrix (remifa_quantize_fp8):
+    rows=True quantizes the m x k matrix a by rows (A operand), rows=False the
+    k x n matrix a by columns (B operand). Returns (codes, scales): uint8
+    tensors [R, ldq] and [R, ldq // 32], row r holding the K-contiguous codes
+    and the E8M0 scale bytes of row / column r (ldq = k rounded up to 128)."""
+    r, k = (a.shape[0], a.shape[1]) if rows else (a.shape[1], a.shape[0])
+    ldq = max(-(-k // 128) * 128, 128)
+    q = torch.zeros((r, ldq), dtype=torch.uint8, device=a.device)
+    sc = torch.full((r, ldq // 32), 127, dtype=torch.uint8, device=a.device)
+    check(lib().remifa_quantize_fp8(_h(handle), dtype_code(a), int(rows), r, k, _p(a), _ld(a),
+                                    _p(q), ldq, _p(sc), ldq // 32), "remifa_quantize_fp8")
+    return q, sc
+
+
+def gemm_fp8(handle, qa: torch.Tensor, sa: torch.Tensor, qb: torch.Tensor, sb: torch.Tensor,
+             k: int, c: torch.Tensor, alpha: float = -1.0, beta: float = 1.0):
+    """c <- alpha A B + beta c from quantize_fp8 operands (remifa_gemm_fp8)."""
+    m, n = c.shape
+    check(lib().remifa_gemm_fp8(_h(handle), dtype_code(c), m, n, k, alpha, _p(qa), qa.stride(0),
+                                _p(sa), sa.stride(0), _p(qb), qb.stride(0), _p(sb), sb.stride(0),
+                                beta, _p(c), _ld(c)), "remifa_gemm_fp8")
 
 
 # ------------------------------------------------------------------ solves

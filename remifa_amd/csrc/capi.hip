@@ -165,7 +165,7 @@ int remifa_gemmex(remifa_handle_t h, int ctype, int in_t, int out_t, int m, int 
                   float alpha, const void* a, int lda, const void* b, int ldb, float beta, void* c,
                   int ldc) {
   if (!h) return -1;
-  if (ctype < REMIFA_FP32 || ctype > REMIFA_BF16_TC32) return -2;
+  if (ctype < REMIFA_FP32 || ctype > REMIFA_FP8_TC32) return -2;
   if (in_t != REMIFA_R_32F && in_t != REMIFA_R_16F) return -3;
   if (out_t != REMIFA_R_32F && out_t != REMIFA_R_16F) return -4;
   if (m < 0) return -5;
@@ -182,6 +182,50 @@ int remifa_gemmex(remifa_handle_t h, int ctype, int in_t, int out_t, int m, int 
   if (st == -4) return -3;
   if (st == -5) return -4;
   return st;
+}
+
+int remifa_quantize_fp8(remifa_handle_t h, int in_t, int rows, int64_t m, int64_t k,
+                        const void* a, int64_t lda, uint8_t* q, int64_t ldq, uint8_t* s,
+                        int64_t lds) {
+  if (!h) return -1;
+  if (in_t != REMIFA_R_32F && in_t != REMIFA_R_16F) return -2;
+  if (m < 0) return -4;
+  if (k < 0) return -5;
+  if (m == 0 || k == 0) return 0;
+  if (!a) return -6;
+  if (lda < (rows ? m : k)) return -7;
+  if (!q || (uintptr_t)q % 16) return -8;
+  if (ldq % 16 || ldq < (k + 31) / 32 * 32) return -9;
+  if (!s) return -10;
+  if (lds < (k + 31) / 32) return -11;
+  return fp8_quantize_launch(h, in_t, rows != 0, m, k, a, lda, q, ldq, s, lds);
+}
+
+int remifa_gemm_fp8(remifa_handle_t h, int out_t, int64_t m, int64_t n, int64_t k, float alpha,
+                    const uint8_t* qa, int64_t ldqa, const uint8_t* sa, int64_t ldsa,
+                    const uint8_t* qb, int64_t ldqb, const uint8_t* sb, int64_t ldsb, float beta,
+                    void* c, int64_t ldc) {
+  if (!h) return -1;
+  if (out_t != REMIFA_R_32F && out_t != REMIFA_R_16F) return -2;
+  if (m < 0) return -3;
+  if (n < 0) return -4;
+  if (k < 0 || k % 32) return -5;
+  if (m == 0 || n == 0) return 0;
+  const int64_t nkb4 = (k + 127) / 128 * 4;
+  if (k > 0) {
+    if (!qa || (uintptr_t)qa % 16) return -7;
+    if (ldqa % 16 || ldqa < k) return -8;
+    if (!sa || (uintptr_t)sa % 4) return -9;
+    if (ldsa % 4 || ldsa < nkb4) return -10;
+    if (!qb || (uintptr_t)qb % 16) return -11;
+    if (ldqb % 16 || ldqb < k) return -12;
+    if (!sb || (uintptr_t)sb % 4) return -13;
+    if (ldsb % 4 || ldsb < nkb4) return -14;
+  }
+  if (!c) return -16;
+  if (ldc < m) return -17;
+  return fp8_gemm_launch(h, out_t, m, n, k, alpha, qa, ldqa, sa, ldsa, qb, ldqb, sb, ldsb, beta, c,
+                         ldc);
 }
 
 }  // extern "C"

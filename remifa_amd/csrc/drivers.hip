@@ -158,16 +158,39 @@ int validate(const remifa_lu_config_t& c, int algo) {
   // bf16 / fp8 updates read operands rounded from the fp16 storage
   auto lowp = [](int t) { return t == REMIFA_BF16_TC32 || t == REMIFA_FP8_TC32; };
   if (c.W == REMIFA_R_32F && (lowp(c.I) || lowp(c.O))) return 1;
-  if (c.I == REMIFA_FP8_TC32 || c.O == REMIFA_FP8_TC32) return 2;  // not built yet
+  // fp8 operands only for the outer updates (one scale per 32-block of
+  // the factors' rows / columns; the inner updates keep I)
+  if (c.I == REMIFA_FP8_TC32) return 1;
   return 0;
 }
 
 }  // namespace
 
 // Workspace needed by lu_nopiv (bytes).
+// FP8 outer updates, LL: e4m3 mirrors of the finished factors, written once
+// per outer block: L8[r * ld8 + k] (row r of L, K-contiguous) and
+// U8[j * ld8 + k] (column j of U), scale bytes SL / SU [. * lds8 + k / 32].
+// RL quantizes each Schur update's operands into the GEMM workspace.
+struct Fp8Mirror {
+  int64_t ld8 = 0, lds8 = 0;
+  uint8_t *l8 = nullptr, *sl = nullptr, *u8 = nullptr, *su = nullptr;
+  static int64_t ld_codes(int64_t n) { return ceil_div(n, 128) * 128; }
+  static size_t bytes(int64_t m, int64_t n) {
+    const int64_t ld8 = ld_codes(n), lds8 = ld8 / 32;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    return al((size_t)m * ld8) + al((size_t)m * lds8) + al((size_t)n * ld8) + al((size_t)n * lds8);
+  }
+};
+
 size_t lu_workspace_bytes(int algo, const remifa_lu_config_t& c, int64_t m, int64_t n, int64_t nb,
                           bool have_cpy) {
   size_t b = (subpanel_table_bytes() + 255) & ~(size_t)255;  // slice table
+  if (c.O == REMIFA_FP8_TC32) {
+    if (algo == REMIFA_LL)
+      b += Fp8Mirror::bytes(m, n);
+    else  // quantized Schur operands (m + n rows of K = nb), taken by the GEMM
+      b += (size_t)(m + n) * (size_t)(Fp8Mirror::ld_codes(nb) * 33 / 32) + 4096;
+  }
   bool need_cpy = c.W == REMIFA_R_32F && (c.I != REMIFA_FP32 || c.O != REMIFA_FP32);
   if (need_cpy && !have_cpy) b += ((size_t)m * n * 2 + 255) & ~(size_t)255;
   if (algo == REMIFA_LL && c.W == REMIFA_R_16F && c.IP == REMIFA_R_32F) {
@@ -185,6 +208,7 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
   if (m < 0) return -4;
   if (n != m) return -5;  // the reference drivers assume square matrices
   if (nb <= 0) return -6;
+  if (cfg && cfg->O == REMIFA_FP8_TC32 && nb % 32) return -6;  // whole 32-blocks per outer block
   if (!d_a && m > 0) return -7;
   if (ldda < std::max<int64_t>(m, 1)) return -8;
   if (d_cpy && ldcpy < std::max<int64_t>(m, 1)) return -11;
@@ -217,8 +241,40 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
     pl = ar.take((size_t)m * nb * 4);
     pu = ar.take((size_t)nb * m * 4);
   }
+  Fp8Mirror f8;
+  const bool fp8 = algo == REMIFA_LL && c.O == REMIFA_FP8_TC32;
+  if (fp8) {
+    f8.ld8 = Fp8Mirror::ld_codes(n);
+    f8.lds8 = f8.ld8 / 32;
+    f8.l8 = (uint8_t*)ar.take((size_t)m * f8.ld8);
+    f8.sl = (uint8_t*)ar.take((size_t)m * f8.lds8);
+    f8.u8 = (uint8_t*)ar.take((size_t)n * f8.ld8);
+    f8.su = (uint8_t*)ar.take((size_t)n * f8.lds8);
+  }
   h->ws_used = ar.off;
   const int64_t nc = (n - 1) / nb + 1;
+  // outer update from the mirrors: C[rows r0.., cols c0..] (M x N) -= L[r0.., 0:K] U[0:K, c0..]
+  auto gemm8 = [&](int out_t, int64_t M, int64_t N, int64_t K, int64_t r0, int64_t c0, void* cc,
+                   int64_t ldcc) -> int {
+    if (M <= 0 || N <= 0 || K <= 0) return 0;
+    return fp8_gemm_launch(h, out_t, M, N, K, -1.0f, f8.l8 + r0 * f8.ld8, f8.ld8,
+                           f8.sl + r0 * f8.lds8, f8.lds8, f8.u8 + c0 * f8.ld8, f8.ld8,
+                           f8.su + c0 * f8.lds8, f8.lds8, 1.0f, cc, ldcc);
+  };
+  // quantize the finished block kk: L rows below its diagonal block, U
+  // columns right of it (from the stored factors, in W precision)
+  auto mirror8 = [&](int64_t ofs, int64_t in) -> int {
+    const int64_t r0 = ofs + in;
+    if (m > r0)
+      RF_CHECK(fp8_quantize_launch(h, W, true, m - r0, in, at(d_a, W, r0, ofs, ldda), ldda,
+                                   f8.l8 + r0 * f8.ld8 + ofs, f8.ld8, f8.sl + r0 * f8.lds8 + ofs / 32,
+                                   f8.lds8));
+    if (n > r0)
+      RF_CHECK(fp8_quantize_launch(h, W, false, n - r0, in, at(d_a, W, ofs, r0, ldda), ldda,
+                                   f8.u8 + r0 * f8.ld8 + ofs, f8.ld8, f8.su + r0 * f8.lds8 + ofs / 32,
+                                   f8.lds8));
+    return 0;
+  };
 
   auto run = [&]() -> int {
     if (algo == REMIFA_LL) {
@@ -234,7 +290,11 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
                                  : at(d_a, W, 0, ofs + in, ldda);
         const int64_t lo = o_copy ? ldc : ldda;
         if (!buffered) {
-          if (ofs > 0) {
+          if (ofs > 0 && fp8) {
+            RF_CHECK(gemm8(W, updm, in, ofs, ofs, ofs, blk, ldda));
+            RF_CHECK(gemm8(W, in, updm - in, ofs, ofs, ofs + in, at(d_a, W, ofs, ofs + in, ldda),
+                           ldda));
+          } else if (ofs > 0) {
             // block column (:239-304) and block row (:311-410), in place
             RF_CHECK(gemm(x, c.O, W, updm, in, ofs, la, lo, ua, lo, blk, ldda));
             if (updm > in)
@@ -245,10 +305,15 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
           // TC update) the slice kernels keep it current (:554-563)
           void* cblk = need_cpy ? at(cpy, REMIFA_R_16F, ofs, ofs, ldc) : nullptr;
           RF_CHECK(inner_ll(x, W, updm, in, blk, ldda, blk, ldda, cblk, ldc, need_cpy));
+          if (fp8) RF_CHECK(mirror8(ofs, in));
         } else {
           // fp16 storage, fp32 inner buffers: pl = block column (updm x in),
           // pu = block row incl. the diagonal block (in x updm)
-          if (c.OP == REMIFA_R_16F && ofs > 0) {
+          if (c.OP == REMIFA_R_16F && ofs > 0 && fp8) {
+            RF_CHECK(gemm8(W, updm, in, ofs, ofs, ofs, blk, ldda));
+            RF_CHECK(gemm8(W, in, updm - in, ofs, ofs, ofs + in, at(d_a, W, ofs, ofs + in, ldda),
+                           ldda));
+          } else if (c.OP == REMIFA_R_16F && ofs > 0) {
             RF_CHECK(gemm(x, c.O, W, updm, in, ofs, la, lo, ua, lo, blk, ldda));
             if (updm > in)
               RF_CHECK(gemm(x, c.O, W, in, updm - in, ofs, la, lo, ua2, lo,
@@ -259,7 +324,10 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
           // the U part right of each tile from pu)
           RF_CHECK(convert_launch(h, W, REMIFA_R_32F, updm, in, blk, ldda, pl, ldpl));
           RF_CHECK(convert_launch(h, W, REMIFA_R_32F, in, updm, blk, ldda, pu, ldpu));
-          if (c.OP == REMIFA_R_32F && ofs > 0) {
+          if (c.OP == REMIFA_R_32F && ofs > 0 && fp8) {
+            RF_CHECK(gemm8(REMIFA_R_32F, updm, in, ofs, ofs, ofs, pl, ldpl));
+            RF_CHECK(gemm8(REMIFA_R_32F, in, updm, ofs, ofs, ofs, pu, ldpu));
+          } else if (c.OP == REMIFA_R_32F && ofs > 0) {
             RF_CHECK(gemm(x, c.O, REMIFA_R_32F, updm, in, ofs, la, lo, ua, lo, pl, ldpl));
             RF_CHECK(gemm(x, c.O, REMIFA_R_32F, in, updm, ofs, la, lo, ua, lo, pu, ldpu));
           }
@@ -267,6 +335,7 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
           // reference passes d_a as the fp16 copy, src/lu_nopiv_ll.cxx:632),
           // so the rounded factors land in A as they are produced
           RF_CHECK(inner_ll(x, REMIFA_R_32F, updm, in, pl, ldpl, pu, ldpu, blk, ldda, true));
+          if (fp8) RF_CHECK(mirror8(ofs, in));
         }
       }
     } else {

@@ -27,6 +27,7 @@
 // fp16-accumulating compute types (FP16, TC16: cublasHgemm / GemmEx 16F) run
 // on a VALU kernel with one correctly rounded fp16 fma per k, in k order.
 #include <algorithm>
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -34,6 +35,7 @@ namespace remifa {
 namespace {
 
 #include "gemm_mfma.hpp"
+#include "gemm_fp8.hpp"
 
 // C = beta*C + sum_z W[z] (slices added in order z = 0, 1, ...)
 template <typename OutT>
@@ -222,6 +224,96 @@ int dispatch_tiles(remifa_handle_t h, int tile, GemmArgs g, int split, bool vec)
 
 }  // namespace
 
+int fp8_quantize_launch(remifa_handle_t h, int in_t, bool rows, int64_t R, int64_t K,
+                        const void* src, int64_t ld, uint8_t* q, int64_t ldq, uint8_t* s,
+                        int64_t lds) {
+  if (R <= 0 || K <= 0) return 0;
+  if (!src || !q || !s) return -6;
+  if (ldq % 16 || (uintptr_t)q % 16 || ldq < ceil_div(K, 32) * 32 || lds < ceil_div(K, 32))
+    return -7;
+  if (ld < (rows ? R : K)) return -5;
+  dim3 grid((unsigned)ceil_div(R, 256), (unsigned)ceil_div(K, 32));
+  timer_begin(h, KC_CONVERT, (double)R * K * ((in_t == REMIFA_R_32F ? 4 : 2) + 1), 8);
+  if (in_t == REMIFA_R_16F) {
+    if (rows)
+      quant_fp8_kernel<half_t, true><<<grid, 256, 0, h->stream>>>(R, K, (const half_t*)src, ld, q,
+                                                                  ldq, s, lds);
+    else
+      quant_fp8_kernel<half_t, false><<<grid, 256, 0, h->stream>>>(R, K, (const half_t*)src, ld,
+                                                                   q, ldq, s, lds);
+  } else if (in_t == REMIFA_R_32F) {
+    if (rows)
+      quant_fp8_kernel<float, true><<<grid, 256, 0, h->stream>>>(R, K, (const float*)src, ld, q,
+                                                                 ldq, s, lds);
+    else
+      quant_fp8_kernel<float, false><<<grid, 256, 0, h->stream>>>(R, K, (const float*)src, ld, q,
+                                                                  ldq, s, lds);
+  } else {
+    timer_end(h);
+    return -4;
+  }
+  timer_end(h);
+  RF_HIP(hipGetLastError());
+  return 0;
+}
+
+int fp8_gemm_launch(remifa_handle_t h, int out_t, int64_t m, int64_t n, int64_t k, float alpha,
+                    const uint8_t* qa, int64_t ldqa, const uint8_t* sa, int64_t ldsa,
+                    const uint8_t* qb, int64_t ldqb, const uint8_t* sb, int64_t ldsb, float beta,
+                    void* c, int64_t ldc) {
+  if (m <= 0 || n <= 0) return 0;
+  if (k < 0 || k % 32) return -3;
+  if (out_t != REMIFA_R_32F && out_t != REMIFA_R_16F) return -5;
+  if (!c || (k > 0 && (!qa || !qb || !sa || !sb))) return -6;
+  const int64_t nkb4 = ceil_div(ceil_div(k, 32), 4) * 4;
+  if (ldqa % 16 || ldqb % 16 || (uintptr_t)qa % 16 || (uintptr_t)qb % 16 || ldqa < k ||
+      ldqb < k || ldsa % 4 || ldsb % 4 || (uintptr_t)sa % 4 || (uintptr_t)sb % 4 ||
+      ldsa < nkb4 || ldsb < nkb4 || ldc < m)
+    return -7;
+  Fp8Args g{};
+  g.M = m, g.N = n, g.K = k;
+  g.A = qa, g.lda = ldqa, g.SA = sa, g.ldsa = ldsa;
+  g.B = qb, g.ldb = ldqb, g.SB = sb, g.ldsb = ldsb;
+  g.alpha = alpha, g.beta = beta, g.C = c, g.ldc = ldc;
+  g.tiles_m = (int)ceil_div(m, 128);
+  g.tiles_n = (int)ceil_div(n, 128);
+  constexpr int BK = 128;
+  int split = h->split_k;
+  const size_t avail = h->ws_bytes > h->ws_used ? h->ws_bytes - h->ws_used : 0;
+  if (split <= 0)
+    split = k > 0 ? split_model(m, n, k, TileGeom{128, 128, 2}, h->num_cu, 7.0e12, BK, avail) : 1;
+  g.kchunk = ceil_div(ceil_div(k > 0 ? k : 1, split), BK) * BK;
+  split = (int)ceil_div(k > 0 ? k : 1, g.kchunk);
+  while (split > 1 && sizeof(float) * (size_t)split * m * n > avail) {
+    split = (split + 1) / 2;
+    g.kchunk = ceil_div(ceil_div(k, split), BK) * BK;
+    split = (int)ceil_div(k, g.kchunk);
+  }
+  if (split > 1) {
+    g.W = (float*)((char*)h->ws + h->ws_used);
+    g.ldw = m;
+  }
+  timer_begin(h, KC_GEMM, 2.0 * (double)m * n * k, (3 << 8) | ((out_t == REMIFA_R_16F) << 4) | T128x128);
+  dim3 grid((unsigned)(g.tiles_m * g.tiles_n), 1, (unsigned)split);
+  if (out_t == REMIFA_R_32F)
+    gemm_fp8_kernel<float><<<grid, 256, 0, h->stream>>>(g);
+  else
+    gemm_fp8_kernel<half_t><<<grid, 256, 0, h->stream>>>(g);
+  int st = hipGetLastError() == hipSuccess ? 0 : REMIFA_ERR_HIP;
+  if (st == 0 && split > 1) {
+    dim3 rg((unsigned)ceil_div(m, 256), (unsigned)(n < 65535 ? n : 65535));
+    if (out_t == REMIFA_R_32F)
+      splitk_reduce_kernel<float><<<rg, 256, 0, h->stream>>>(m, n, split, g.W, g.ldw, beta,
+                                                             (float*)c, ldc);
+    else
+      splitk_reduce_kernel<half_t><<<rg, 256, 0, h->stream>>>(m, n, split, g.W, g.ldw, beta,
+                                                              (half_t*)c, ldc);
+    if (hipGetLastError() != hipSuccess) st = REMIFA_ERR_HIP;
+  }
+  timer_end(h);
+  return st;
+}
+
 int gemm_launch(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t m, int64_t n, int64_t k,
                 float alpha, const void* a, int64_t lda, const void* b, int64_t ldb, float beta,
                 void* c, int64_t ldc) {
@@ -236,6 +328,38 @@ int gemm_launch(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t m, in
     case REMIFA_FP32: opk = OPK_F32; if (in_t != REMIFA_R_32F) return -4; break;
     case REMIFA_TC32: opk = OPK_F16; if (in_t != REMIFA_R_16F) return -4; break;
     case REMIFA_BF16_TC32: opk = OPK_BF16; if (in_t != REMIFA_R_16F) return -4; break;
+    case REMIFA_FP8_TC32: {
+      // quantize A by rows and B by columns into the free workspace, then
+      // the scaled matrix-core GEMM (K padded to whole 32-blocks with zeros)
+      if (in_t != REMIFA_R_16F && in_t != REMIFA_R_32F) return -4;
+      if (out_t != REMIFA_R_32F && out_t != REMIFA_R_16F) return -5;
+      const int64_t kp = ceil_div(k > 0 ? k : 0, 32) * 32;
+      const int64_t ldq = std::max<int64_t>(ceil_div(kp, 128) * 128, 128);
+      const int64_t lds = ldq / 32;
+      auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+      const size_t qbytes = al((size_t)m * ldq) + al((size_t)m * lds) + al((size_t)n * ldq) +
+                            al((size_t)n * lds);
+      if (h->ws_used == 0 && !h->ws_user) {
+        const size_t part = std::min<size_t>((size_t)256 << 20, (size_t)16 * 4 * m * n);
+        RF_CHECK(ensure_workspace(h, qbytes + part));
+      }
+      if (h->ws_used + qbytes > h->ws_bytes) return REMIFA_ERR_WORKSPACE;
+      char* base = (char*)h->ws + h->ws_used;
+      uint8_t* qa = (uint8_t*)base;
+      uint8_t* sa = qa + al((size_t)m * ldq);
+      uint8_t* qb = sa + al((size_t)m * lds);
+      uint8_t* sb = qb + al((size_t)n * ldq);
+      if (kp > 0) {
+        RF_CHECK(fp8_quantize_launch(h, in_t, true, m, k, a, lda, qa, ldq, sa, lds));
+        RF_CHECK(fp8_quantize_launch(h, in_t, false, n, k, b, ldb, qb, ldq, sb, lds));
+      }
+      const size_t saved = h->ws_used;
+      h->ws_used += qbytes;
+      const int st = fp8_gemm_launch(h, out_t, m, n, kp, alpha, qa, ldq, sa, lds, qb, ldq, sb, lds,
+                                     beta, c, ldc);
+      h->ws_used = saved;
+      return st;
+    }
     case REMIFA_FP16:
     case REMIFA_TC16: {
       if (in_t != REMIFA_R_16F) return -4;

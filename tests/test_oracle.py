@@ -314,6 +314,51 @@ def test_gemm_fp16_accumulation_brute_force():
             assert got[i, j] == np.float32(want)
 
 
+def test_quantise_e4m3_known_answers():
+    """Hand-derived codes of the FP8TC32 operand format: scale 2^(floor(log2
+    amax) - 8) per 32-block, e4m3 nearest even, saturation at 448."""
+    row = np.zeros(70, np.float32)
+    row[:6] = [1.0, 0.75, 2.0**-17, 3 * 2.0**-19, -0.5, 5 * 2.0**-19]
+    row[32:34] = [1.9, -1.0]   # block 1: amax 1.9 -> 1.9 * 2^8 = 486.4 saturates
+    row[64:66] = [3.0, 0.25]   # block 2 (partial, 6 long)
+    codes, sexp = orc.quantise_e4m3(row[None, :], rows=True)
+    assert sexp.tolist() == [[-8, -8, -7]]
+    # 1.0 * 2^8 = 256 = 1.0 * 2^8 -> e 15 m 0; 0.75 * 2^8 = 192 = 1.5 * 2^7 -> e 14 m 4
+    # 2^-17 * 2^8 = 2^-9 -> subnormal m 1; 3 * 2^-11 = 0.75 * 2^-9 -> rounds to 2^-9
+    # -0.5 * 2^8 = -128 -> sign, e 14 m 0; 5 * 2^-11 = 1.25 * 2^-9 -> 2^-9 (nearest)
+    assert codes[0, :6].tolist() == [0x78, 0x74, 0x01, 0x01, 0xF0, 0x01]
+    # 486.4 -> 448 (0x7E); -256 -> 0xF8
+    assert codes[0, 32:34].tolist() == [0x7E, 0xF8]
+    # block 2: amax 3 -> 2^(1 - 8): 3 * 2^7 = 384 = 1.5 * 2^8 -> e 15 m 4; 0.25 * 2^7 = 32 -> e 12
+    assert codes[0, 64:66].tolist() == [0x7C, 0x60]
+    assert np.all(codes[0, 6:32] == 0) and np.all(codes[0, 66:] == 0)
+    # an all-zero block has scale 2^0
+    z = orc.quantise_e4m3(np.zeros((1, 40), np.float32))
+    assert z[1].tolist() == [[0, 0]] and not z[0].any()
+    # rows of A and columns of the transpose are the same blocks
+    rng = np.random.default_rng(4)
+    a = (rng.standard_normal((7, 100)) * np.exp2(rng.integers(-20, 10, (7, 1)))).astype(np.float32)
+    ca, ea = orc.quantise_e4m3(a, rows=True)
+    cb, eb = orc.quantise_e4m3(np.asfortranarray(a.T), rows=False)
+    assert np.array_equal(ca, cb) and np.array_equal(ea, eb)
+
+
+def test_quantise_e4m3_elements_match_torch_float8():
+    """Element rounding against torch's float8_e4m3fn cast (nearest even)
+    where the scaled value is within the finite range (torch does not
+    saturate: values above 448 are the oracle's saturation case)."""
+    import torch
+    rng = np.random.default_rng(12)
+    a = (rng.standard_normal((64, 256)) * np.exp2(rng.integers(-12, 12, (64, 8)).repeat(32, 1))).astype(np.float32)
+    codes, sexp = orc.quantise_e4m3(a, rows=True)
+    scaled = a * np.exp2(-sexp.repeat(32, axis=1).astype(np.float64))
+    want = torch.from_numpy(scaled.astype(np.float32)).to(torch.float8_e4m3fn).view(torch.uint8).numpy()
+    inr = np.abs(scaled) <= 448
+    assert np.array_equal(codes[inr], want[inr])
+    assert np.all((codes[~inr] & 0x7F) == 0x7E)
+    assert inr.mean() > 0.9
+
+
 def test_gemm_fp8_per_block_scale_bound():
     rng = np.random.default_rng(9)
     m, n, k = 32, 32, 128
