@@ -53,6 +53,10 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-fp32", action="store_true")
+    p.add_argument("--update", choices=["fp16", "bf16", "fp8"], default="fp16",
+                   help="outer-update operands of the timed factorization")
+    p.add_argument("--no-variants", action="store_true",
+                   help="skip the BF16 / FP8 update variants and the nrhs=16 refinement")
     p.add_argument("--workload", choices=["lu", "batched", "autograd"], default="lu",
                    help="lu: the headline n=32768 LU + IR (default); batched: 20000 LUs + "
                         "solves of orders U[16,512]; autograd: X = A^-1 B forward + backward")
@@ -273,10 +277,11 @@ def main():
     X = torch.zeros((a.nrhs, n), dtype=torch.float64, device=dev).t()
     A16 = rf.colmajor_empty(n, n, dtype=torch.float16, device=dev)
     algo = rf.LL if a.algo == "ll" else rf.RL
+    o_upd = {"fp16": rf.TC32, "bf16": rf.BF16_TC32, "fp8": rf.FP8_TC32}[a.update]
     if algo == rf.LL:
-        cfg16 = rf.api.make_config(W=rf.R_16F, I=rf.TC32, O=rf.TC32, OP=rf.R_32F, ib=a.ib)
+        cfg16 = rf.api.make_config(W=rf.R_16F, I=rf.TC32, O=o_upd, OP=rf.R_32F, ib=a.ib)
     else:
-        cfg16 = rf.api.make_config(W=rf.R_16F, I=rf.TC32, O=rf.TC32, ib=a.ib)
+        cfg16 = rf.api.make_config(W=rf.R_16F, I=rf.TC32, O=o_upd, ib=a.ib)
     info = torch.zeros(1, dtype=torch.int32, device=dev)
     torch.cuda.synchronize()
 
@@ -369,15 +374,18 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp16 (fp32 accumulate); fp64 residual",
+        "dtype": {"fp16": "fp16", "bf16": "bf16 outer / fp16 inner",
+                  "fp8": "e4m3 (per-32-block scaled) outer / fp16 inner"}[a.update]
+                 + " (fp32 accumulate); fp64 residual",
         "data": f"synthetic: seeded minstd_rand0 uniform(-1,1), a_ii = {a.alpha}*sum|a_ij|, "
                 f"generated on the GPU",
-        "config": {"workload": f"no-pivot LU n={n} fp16 storage, TC32 MFMA updates, "
-                               f"+ FP64-residual IR to 1e-14, nrhs={a.nrhs}",
+        "config": {"workload": f"no-pivot LU n={n} fp16 storage, "
+                               f"{ {'fp16': 'TC32', 'bf16': 'BF16_TC32', 'fp8': 'FP8_TC32'}[a.update]} "
+                               f"MFMA outer updates, + FP64-residual IR to 1e-14, nrhs={a.nrhs}",
                    "n": n, "nb": nb, "ib": a.ib, "algo": a.algo, "alpha": a.alpha,
                    "nrhs": a.nrhs, "seed": a.seed,
-                   "precision": "W=f16 I=O=TC32 OP=IP=OF=IF=f32" if algo == rf.LL
-                   else "W=f16 I=O=TC32"},
+                   "precision": (f"W=f16 I=TC32 O={ {'fp16': 'TC32', 'bf16': 'BF16_TC32', 'fp8': 'FP8_TC32'}[a.update]}"
+                                 + (" OP=IP=OF=IF=f32" if algo == rf.LL else ""))},
         "factor_ms": round(t_fac, 3),
         "ir_ms": round(t_ir, 3),
         "ir_iters": iters,
@@ -446,6 +454,51 @@ def main():
         out["speedup_tts_vs_fp32_lu_fp64_ir"] = round((lu32 + ir32) / tts, 3)
         out["speedup_factor_vs_fp32"] = round(lu32 / t_fac, 3)
         del A32
+
+    if not a.no_variants:
+        # the same factorization with BF16 and FP8 (per-32-block scaled e4m3)
+        # outer updates, then FP64-residual refinement to 1e-14; and the
+        # nrhs = 16 refinement on the FP16 factors
+        var = {}
+        for name, O in (("bf16", rf.BF16_TC32), ("fp8", rf.FP8_TC32)):
+            if algo == rf.LL:
+                cfgv = rf.api.make_config(W=rf.R_16F, I=rf.TC32, O=O, OP=rf.R_32F, ib=a.ib)
+            else:
+                cfgv = rf.api.make_config(W=rf.R_16F, I=rf.TC32, O=O, ib=a.ib)
+            res_v = []
+            for k in range(2):  # one warmup, one timed
+                rf.convert(h, A, A16)
+                X.zero_()
+                e0, e1, e2 = ev(), ev(), ev()
+                e0.record(s)
+                rf.lu_nopiv(h, A16, nb, algo=algo, cfg=cfgv, info=info)
+                e1.record(s)
+                it_v, be_v, _ = rf.ir_solve(h, A, A16, B, X, tol=1e-14, maxit=50)
+                e2.record(s)
+                torch.cuda.synchronize()
+                res_v.append((e0.elapsed_time(e1), e1.elapsed_time(e2), it_v, be_v))
+            fv, iv, itv, bev = res_v[-1]
+            var[name] = {"factor_ms": round(fv, 3), "tflops": round(flops / (fv * 1e-3) / 1e12, 2),
+                         "ir_ms": round(iv, 3), "ir_iters": itv, "backward_error": bev,
+                         "time_to_solution_ms": round(fv + iv, 3)}
+        rf.convert(h, A, A16)
+        rf.lu_nopiv(h, A16, nb, algo=algo, cfg=cfg16, info=info)
+        b16f = rf.colmajor_empty(n, 16, device=dev)
+        rf.gen_uniform(h, b16f, seed=a.seed + 2)
+        B16 = torch.zeros((16, n), dtype=torch.float64, device=dev).t()
+        B16.copy_(b16f)
+        X16 = torch.zeros((16, n), dtype=torch.float64, device=dev).t()
+        rf.ir_solve(h, A, A16, B16, X16, tol=1e-14, maxit=30)  # warmup
+        X16.zero_()
+        e0, e1 = ev(), ev()
+        e0.record(s)
+        it16, be16, _ = rf.ir_solve(h, A, A16, B16, X16, tol=1e-14, maxit=30)
+        e1.record(s)
+        torch.cuda.synchronize()
+        var["fp16_nrhs16"] = {"ir_ms": round(e0.elapsed_time(e1), 3), "ir_iters": it16,
+                              "backward_error": be16}
+        out["update_variants"] = var
+        del b16f, B16, X16
 
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.cpu_seconds, a.cpu_n)

@@ -51,18 +51,22 @@ __device__ __forceinline__ v8i32 join16(uint4 lo, uint4 hi) {
                  (int)hi.x, (int)hi.y, (int)hi.z, (int)hi.w};
 }
 
-// scale bytes of a 128-long K slab (four 32-blocks) of row r: blocks past
-// the K range, and rows past the matrix, read as 2^0 (their codes are 0)
-__device__ __forceinline__ uint32_t slab_scales(const uint8_t* s, int64_t lds, int64_t r,
-                                                int64_t nrows, int64_t kb0, int64_t nkb) {
-  const bool rok = r < nrows;
-  const uint32_t w = *(const uint32_t*)(s + (rok ? r : 0) * lds + kb0);
-  const int64_t valid = rok ? (nkb - kb0 < 4 ? nkb - kb0 : 4) : 0;
-  uint32_t out = w;
-#pragma unroll
-  for (int b = 0; b < 4; ++b)
-    if (b >= valid) out = (out & ~(0xFFu << (8 * b))) | (127u << (8 * b));
-  return out;
+// scale bytes of a 128-long K slab (four 32-blocks) of row r, in two steps
+// so that nothing waits for the load before the matrix-core work: the word
+// (from a clamped, in-bounds address when out of range) and the number of
+// its valid blocks; blocks past the K range, and rows past the matrix, then
+// read as 2^0 (their codes are 0)
+__device__ __forceinline__ uint32_t scale_word(const uint8_t* s, int64_t lds, int64_t r,
+                                               int64_t nrows, int64_t kb0, int64_t nkb,
+                                               int& valid) {
+  const bool ok = r < nrows && kb0 < nkb;
+  valid = ok ? (int)(nkb - kb0 < 4 ? nkb - kb0 : 4) : 0;
+  return *(const uint32_t*)(s + (ok ? r * lds + kb0 : 0));
+}
+__device__ __forceinline__ uint32_t scale_mask(uint32_t w, int valid) {
+  // bytes >= valid -> 127
+  const uint32_t keep = valid >= 4 ? 0xFFFFFFFFu : ((1u << (8 * valid)) - 1u);
+  return (w & keep) | (0x7F7F7F7Fu & ~keep);
 }
 
 template <typename OutT>
@@ -96,7 +100,8 @@ __global__ void __launch_bounds__(256, 2) gemm_fp8_kernel(Fp8Args g) {
   uint4 ra[4], rb[4];
   bool oka[4], okb[4];
   uint32_t nsa[2], nsb[2];  // scale words of the slab being loaded
-  uint32_t csa[2], csb[2];  // and of the slab being computed
+  int nva[2], nvb[2];       // and their valid block counts
+  uint32_t csa[2], csb[2];  // scale words of the slab being computed
 
   auto load_slab = [&](int kt) {
     const int64_t k0 = kbeg + (int64_t)kt * BK;
@@ -114,8 +119,8 @@ __global__ void __launch_bounds__(256, 2) gemm_fp8_kernel(Fp8Args g) {
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-      nsa[i] = slab_scales(g.SA, g.ldsa, m0 + wm * 64 + i * 32 + l32, g.M, k0 / 32, nkb);
-      nsb[i] = slab_scales(g.SB, g.ldsb, n0 + wn * 64 + i * 32 + l32, g.N, k0 / 32, nkb);
+      nsa[i] = scale_word(g.SA, g.ldsa, m0 + wm * 64 + i * 32 + l32, g.M, k0 / 32, nkb, nva[i]);
+      nsb[i] = scale_word(g.SB, g.ldsb, n0 + wn * 64 + i * 32 + l32, g.N, k0 / 32, nkb, nvb[i]);
     }
   };
   auto store_tiles = [&](int buf) {
@@ -128,8 +133,8 @@ __global__ void __launch_bounds__(256, 2) gemm_fp8_kernel(Fp8Args g) {
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-      csa[i] = nsa[i];
-      csb[i] = nsb[i];
+      csa[i] = scale_mask(nsa[i], nva[i]);
+      csb[i] = scale_mask(nsb[i], nvb[i]);
     }
   };
 
@@ -175,10 +180,15 @@ __global__ void __launch_bounds__(256, 2) gemm_fp8_kernel(Fp8Args g) {
   __syncthreads();
   for (int kt = 0; kt < nk; ++kt) {
     const int buf = kt & 1;
-    const bool more = kt + 1 < nk;
-    if (more) load_slab(kt + 1);
+    // straight-line body (after the last slab the loads are masked, clamped
+    // to in-bounds addresses, and land in the buffer nobody reads again): a
+    // branch around the stores lets the compiler sink the matrix-core work
+    // below them, behind the waits for the loads just issued
+    load_slab(kt + 1);
+    __builtin_amdgcn_sched_barrier(0);  // the loads go out first
     compute(buf);
-    if (more) store_tiles(buf ^ 1);
+    __builtin_amdgcn_sched_barrier(0);
+    store_tiles(buf ^ 1);
     __syncthreads();
   }
 
