@@ -53,6 +53,9 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-fp32", action="store_true")
+    p.add_argument("--pmc-calibrate", action="store_true",
+                   help="first copy 256 MiB device to device (a dispatch of known HBM "
+                        "traffic, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     p.add_argument("--update", choices=["fp16", "bf16", "fp8"], default="fp16",
                    help="outer-update operands of the timed factorization")
     p.add_argument("--no-variants", action="store_true",
@@ -265,6 +268,13 @@ def main():
     import remifa_amd as rf
 
     dev = torch.device("cuda:0")
+    if a.pmc_calibrate:
+        src = torch.empty(64 << 20, dtype=torch.float32, device=dev).uniform_()
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)  # reads 256 MiB, writes 256 MiB
+        torch.cuda.synchronize()
+        del src, dst
     n, nb = a.n, a.nb
     h = rf.Handle()
     s = h.stream

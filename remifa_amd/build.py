@@ -44,9 +44,9 @@ def _digest(path: str) -> str:
     return h.hexdigest()[:16]
 
 
-def _compile(src: str) -> str:
+def _compile(src: str, force: bool = False) -> str:
     obj = os.path.join(BUILD, os.path.basename(src) + "." + _digest(src) + ".o")
-    if not os.path.exists(obj):
+    if force or not os.path.exists(obj):
         cmd = ["hipcc", *FLAGS, "-c", src, "-o", obj + ".tmp"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
@@ -55,11 +55,16 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(verbose: bool = True) -> str:
+def build(verbose: bool = True, force: bool = False) -> str:
+    """Compile every source (force) or the ones whose object is not cached
+    under _build/ (keyed by the sources, headers and flags), then link."""
     os.makedirs(BUILD, exist_ok=True)
     srcs = _sources()
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda s: _compile(s, force), srcs))
+    for f in os.listdir(BUILD):  # drop objects of older sources
+        if os.path.join(BUILD, f) not in objs and f.endswith(".o"):
+            os.remove(os.path.join(BUILD, f))
     tmp = LIB + ".tmp"
     cmd = ["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -107,12 +107,20 @@ struct Sem {
 // RL: after it) run as runtime loops over LDS copies of the operands.
 constexpr int TP = 8;  // panel width = columns per wavefront
 
-struct TileLds {
-  float Lp[2][SW][TP + 1];  // multipliers of the current panel (double-buffered)
-  float Uo[SW][SW + PD];    // op-rounded rows of U (row k published at pivot k)
-  float Xo[SW][SW + 1];     // op-rounded multipliers
-  float R[SW];              // reciprocals of the pivots
+// Row pitches are 16-byte multiples so that a lane publishes its 8 values
+// with two 16-byte stores; 48- and 272-byte pitches keep 16 consecutive
+// lanes on distinct banks.
+struct __attribute__((aligned(16))) TileLds {
+  __attribute__((aligned(16))) float Lp[2][SW][TP + 4];  // multipliers of the current panel
+  __attribute__((aligned(16))) float Uo[SW][SW + PD];    // op-rounded rows of U
+  __attribute__((aligned(16))) float Xo[SW][SW + PD];    // op-rounded multipliers
+  float R[SW];                                           // reciprocals of the pivots
 };
+
+__device__ __forceinline__ void st8(float* p, const float* v) {
+  *(float4*)p = make_float4(v[0], v[1], v[2], v[3]);
+  *(float4*)(p + 4) = make_float4(v[4], v[5], v[6], v[7]);
+}
 
 // value of v in lane `lane` (wave-uniform) for every lane of the wavefront
 __device__ __forceinline__ float bcast_lane(float v, int lane) {
@@ -169,7 +177,9 @@ __global__ void __launch_bounds__(512) tile_kernel(int sw, int ib, void* __restr
     }
     if (w == g) {
       // factor the panel: pivot by pivot inside this wavefront; lane k's
-      // row is read by every lane with readlane (no LDS round trip)
+      // row is read by every lane with readlane (no LDS round trip). Only
+      // the arithmetic is on this chain: the panel is published to LDS
+      // afterwards, by all lanes at once.
 #pragma unroll
       for (int kk = 0; kk < TP; ++kk) {
         const int k = p0 + kk;
@@ -177,19 +187,26 @@ __global__ void __launch_bounds__(512) tile_kernel(int sw, int ib, void* __restr
 #pragma unroll
         for (int jj = kk; jj < TP; ++jj) u[jj] = bcast_lane(x[jj], k);
         const float r = Sm::rcp(u[kk]);
-        if (i == k) {
-          S.R[k] = r;
-#pragma unroll
-          for (int jj = kk; jj < TP; ++jj) S.Uo[k][cb + jj] = Sm::op(x[jj]);
-        }
         if (i > k) {
           const float lk = Sm::mul(x[kk], r);
           x[kk] = lk;
-          S.Lp[g & 1][i][kk] = lk;
-          S.Xo[i][k] = Sm::op(lk);
 #pragma unroll
           for (int jj = kk + 1; jj < TP; ++jj) x[jj] = Sm::fnma(lk, u[jj], x[jj]);
         }
+      }
+      // multipliers (rows below each pivot; the other entries are not read)
+      float o[TP];
+#pragma unroll
+      for (int kk = 0; kk < TP; ++kk) o[kk] = Sm::op(x[kk]);
+      st8(&S.Lp[g & 1][i][0], x);
+      st8(&S.Xo[i][p0], o);
+      if (i >= p0 && i < p0 + TP) {
+        // this lane's row of U in the panel and its pivot's reciprocal
+        st8(&S.Uo[i][cb], o);
+        float d = x[0];
+#pragma unroll
+        for (int kk = 1; kk < TP; ++kk) d = (i - p0 == kk) ? x[kk] : d;
+        S.R[i] = Sm::rcp(d);
       }
     }
     __syncthreads();
@@ -206,14 +223,17 @@ __global__ void __launch_bounds__(512) tile_kernel(int sw, int ib, void* __restr
         float u[TP];
 #pragma unroll
         for (int jj = 0; jj < TP; ++jj) u[jj] = bcast_lane(x[jj], k);
-        if (i == k) {
-#pragma unroll
-          for (int jj = 0; jj < TP; ++jj) S.Uo[k][cb + jj] = Sm::op(x[jj]);
-        }
         if (i > k && upd) {
 #pragma unroll
           for (int jj = 0; jj < TP; ++jj) x[jj] = Sm::fnma(lp[kk], u[jj], x[jj]);
         }
+      }
+      // rows p0 .. p0 + 7 of these columns are final: publish them
+      if (i >= p0 && i < p0 + TP) {
+        float o[TP];
+#pragma unroll
+        for (int jj = 0; jj < TP; ++jj) o[jj] = Sm::op(x[jj]);
+        st8(&S.Uo[i][cb], o);
       }
       if (RL && p0 + TP == kend && cb >= kend) {
         // block complete: right-looking rank-ib update of the trailing part
@@ -238,23 +258,35 @@ __global__ void __launch_bounds__(512) tile_kernel(int sw, int ib, void* __restr
   // store the factors and publish the table: D, op(D) (multipliers and row
   // entries op-rounded), the transposes, the reciprocals
   if (i < sw) {
+    if (cb + TP <= sw) {  // the whole panel is inside the slice
 #pragma unroll
-    for (int jj = 0; jj < TP; ++jj) {
-      const int j = cb + jj;
-      if (j < sw) {
-        l[i + (int64_t)j * ldl] = (T)x[jj];
-        if (cp) cp[i + (int64_t)j * ldcp] = (half_t)x[jj];
+      for (int jj = 0; jj < TP; ++jj) l[i + (int64_t)(cb + jj) * ldl] = (T)x[jj];
+      if (cp) {
+#pragma unroll
+        for (int jj = 0; jj < TP; ++jj) cp[i + (int64_t)(cb + jj) * ldcp] = (half_t)x[jj];
+      }
+    } else {
+#pragma unroll
+      for (int jj = 0; jj < TP; ++jj) {
+        const int j = cb + jj;
+        if (j < sw) {
+          l[i + (int64_t)j * ldl] = (T)x[jj];
+          if (cp) cp[i + (int64_t)j * ldcp] = (half_t)x[jj];
+        }
       }
     }
   }
+  {
+    float o[TP];
 #pragma unroll
-  for (int jj = 0; jj < TP; ++jj) {
-    const int j = cb + jj;
-    const float o = Sm::op(x[jj]);
-    tab[T_D + i * SW + j] = x[jj];
-    tab[T_DT + j * SW + i] = x[jj];
-    tab[T_DOP + i * SW + j] = o;
-    tab[T_DOPT + j * SW + i] = o;
+    for (int jj = 0; jj < TP; ++jj) o[jj] = Sm::op(x[jj]);
+    st8(tab + T_D + i * SW + cb, x);  // row i: 32 contiguous bytes per lane
+    st8(tab + T_DOP + i * SW + cb, o);
+#pragma unroll
+    for (int jj = 0; jj < TP; ++jj) {  // transposes: coalesced across lanes
+      tab[T_DT + (cb + jj) * SW + i] = x[jj];
+      tab[T_DOPT + (cb + jj) * SW + i] = o[jj];
+    }
   }
   __syncthreads();
   if (w == 0) {
