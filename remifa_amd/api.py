@@ -148,6 +148,9 @@ def gemm_variant_name(v: int) -> str:
     if opk == 3:
         return (f"gemm_fp8_kernel<{'_Float16' if out16 else 'float'}> "
                 f"(scaled e4m3 operands, 128x128 tile)")
+    if tile == 8:
+        return (f"gemm_pair_kernel<{opk}, {'_Float16' if out16 else 'float'}> "
+                f"({_GEMM_OPK.get(opk, '?')} operands, 256x32 + 32x256 tiles)")
     return (f"gemm_mfma_kernel<{opk}, {'_Float16' if out16 else 'float'}, {bm}, {bn}, {wm}, {wn}>"
             f" ({_GEMM_OPK.get(opk, '?')} operands, {bm}x{bn} tile)")
 

@@ -114,13 +114,16 @@ int inner_ll(Ctx& x, int T, int64_t m, int64_t n, void* l, int64_t ldl, void* u,
       const int64_t lla = tc_from_copy ? ldcp : ldl;
       const void* ub = tc_from_copy ? at(cpy, REMIFA_R_16F, 0, so, ldcp) : at(u, T, 0, so, ldu);
       const int64_t lub = tc_from_copy ? ldcp : ldu;
-      // columns of the slice (:223-246)
-      RF_CHECK(gemm(x, I, T, cm, sw, so, la, lla, ub, lub, at(l, T, so, so, ldl), ldl));
-      // rows of the slice, right of its diagonal tile (:249-301)
+      // columns of the slice (:223-246) and its rows right of the diagonal
+      // tile (:249-301): one launch when both take the narrow tiles
       if (cm > sw) {
         const void* ub2 = tc_from_copy ? at(cpy, REMIFA_R_16F, 0, so + sw, ldcp)
                                        : at(u, T, 0, so + sw, ldu);
-        RF_CHECK(gemm(x, I, T, sw, cm - sw, so, la, lla, ub2, lub, at(u, T, so, so + sw, ldu), ldu));
+        RF_CHECK(gemm_pair_launch(x.h, I, operand_type(I), T, cm, sw, so, la, lla, ub, lub,
+                                  at(l, T, so, so, ldl), ldl, sw, cm - sw, la, lla, ub2, lub,
+                                  at(u, T, so, so + sw, ldu), ldu));
+      } else {
+        RF_CHECK(gemm(x, I, T, cm, sw, so, la, lla, ub, lub, at(l, T, so, so, ldl), ldl));
       }
     }
     RF_CHECK(subpanel_launch(x.h, T, I, false, (int)x.ib, cm, sw, at(l, T, so, so, ldl), ldl,

@@ -222,7 +222,96 @@ int dispatch_tiles(remifa_handle_t h, int tile, GemmArgs g, int split, bool vec)
   return REMIFA_ERR_UNSUPPORTED;
 }
 
+template <int OPK, typename OutT>
+int launch_pair(remifa_handle_t h, GemmArgs g0, GemmArgs g1, bool vec) {
+  // 32-deep slabs: 49 KB of LDS, three workgroups per CU, so both halves of
+  // the pair (256 + 256 workgroups at the LU's shapes) run in one round
+  constexpr int BK = OPK == OPK_F32 ? 16 : 32;
+  g0.tiles_m = (int)ceil_div(g0.M, 256);
+  g0.tiles_n = (int)ceil_div(g0.N, 32);
+  g1.tiles_m = (int)ceil_div(g1.M, 32);
+  g1.tiles_n = (int)ceil_div(g1.N, 256);
+  const int nt0 = g0.tiles_m * g0.tiles_n;
+  dim3 grid((unsigned)(nt0 + g1.tiles_m * g1.tiles_n));
+  if (vec)
+    gemm_pair_kernel<OPK, OutT, BK, true><<<grid, 256, 0, h->stream>>>(g0, g1, nt0);
+  else
+    gemm_pair_kernel<OPK, OutT, BK, false><<<grid, 256, 0, h->stream>>>(g0, g1, nt0);
+  RF_HIP(hipGetLastError());
+  return 0;
+}
+
 }  // namespace
+
+// The two slice updates of the inner LL factorization in one launch when
+// they take the narrow tiles without split-K (C0 -= A0 B0 with N0 <= 32 per
+// tile column, C1 -= A1 B1 with M1 <= 64); otherwise two gemm_launch calls.
+int gemm_pair_launch(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t m0, int64_t n0,
+                     int64_t k, const void* a0, int64_t lda0, const void* b0, int64_t ldb0,
+                     void* c0, int64_t ldc0, int64_t m1, int64_t n1, const void* a1, int64_t lda1,
+                     const void* b1, int64_t ldb1, void* c1, int64_t ldc1) {
+  int opk = -1;
+  if (ctype == REMIFA_FP32 && in_t == REMIFA_R_32F) opk = OPK_F32;
+  if (ctype == REMIFA_TC32 && in_t == REMIFA_R_16F) opk = OPK_F16;
+  if (ctype == REMIFA_BF16_TC32 && in_t == REMIFA_R_16F) opk = OPK_BF16;
+  const size_t es = (opk == OPK_F32) ? 4 : 2;
+  const int chunk = 16 / (int)es;
+  auto vec_ok = [&](const void* a, int64_t lda, const void* b, int64_t ldb, int64_t m) {
+    return ((uintptr_t)a % 16 == 0) && ((uintptr_t)b % 16 == 0) && (lda % chunk == 0) &&
+           (ldb % chunk == 0) && (m % chunk == 0) && (k % chunk == 0);
+  };
+  // the rule table / overrides must pick the narrow tiles without split-K
+  auto rule = [&](int64_t m, int64_t n, int& tile, int& split) {
+    tile = choose_tile(opk, m, n);
+    split = 0;
+    for (const auto& r : h->gemm_rules)
+      if ((r[0] < 0 || r[0] == opk) && m >= r[1] && m <= r[2] && n >= r[3] && n <= r[4]) {
+        tile = (int)r[5];
+        split = (int)r[6];
+        break;
+      }
+  };
+  int t0, s0, t1, s1;
+  if (opk >= 0) {
+    rule(m0, n0, t0, s0);
+    rule(m1, n1, t1, s1);
+  }
+  const bool v0 = opk >= 0 && vec_ok(a0, lda0, b0, ldb0, m0);
+  const bool v1 = opk >= 0 && vec_ok(a1, lda1, b1, ldb1, m1);
+  const bool pair = opk >= 0 && k > 0 && m0 > 0 && n0 > 0 && m1 > 0 && n1 > 0 && t0 == T256x32 &&
+                    t1 == T32x256 && s0 <= 1 && s1 <= 1 && h->split_k <= 0 && h->gemm_tile < 0 &&
+                    v0 == v1 && (out_t == REMIFA_R_32F || out_t == REMIFA_R_16F);
+  if (!pair) {
+    RF_CHECK(gemm_launch(h, ctype, in_t, out_t, m0, n0, k, -1.0f, a0, lda0, b0, ldb0, 1.0f, c0,
+                         ldc0));
+    return gemm_launch(h, ctype, in_t, out_t, m1, n1, k, -1.0f, a1, lda1, b1, ldb1, 1.0f, c1,
+                       ldc1);
+  }
+  auto args = [&](int64_t m, int64_t n, const void* a, int64_t lda, const void* b, int64_t ldb,
+                  void* c, int64_t ldc) {
+    GemmArgs g{};
+    g.M = m, g.N = n, g.K = k, g.lda = lda, g.ldb = ldb, g.ldc = ldc;
+    g.alpha = -1.0f, g.beta = 1.0f, g.A = a, g.B = b, g.C = c;
+    g.kchunk = k;  // one K slice
+    return g;
+  };
+  GemmArgs g0 = args(m0, n0, a0, lda0, b0, ldb0, c0, ldc0);
+  GemmArgs g1 = args(m1, n1, a1, lda1, b1, ldb1, c1, ldc1);
+  timer_begin(h, KC_GEMM, 2.0 * (double)k * (m0 * n0 + m1 * n1),
+              (opk << 8) | ((out_t == REMIFA_R_16F) << 4) | 8);
+  int st;
+  if (opk == OPK_F32)
+    st = out_t == REMIFA_R_32F ? launch_pair<OPK_F32, float>(h, g0, g1, v0)
+                               : launch_pair<OPK_F32, half_t>(h, g0, g1, v0);
+  else if (opk == OPK_F16)
+    st = out_t == REMIFA_R_32F ? launch_pair<OPK_F16, float>(h, g0, g1, v0)
+                               : launch_pair<OPK_F16, half_t>(h, g0, g1, v0);
+  else
+    st = out_t == REMIFA_R_32F ? launch_pair<OPK_BF16, float>(h, g0, g1, v0)
+                               : launch_pair<OPK_BF16, half_t>(h, g0, g1, v0);
+  timer_end(h);
+  return st;
+}
 
 int fp8_quantize_launch(remifa_handle_t h, int in_t, bool rows, int64_t R, int64_t K,
                         const void* src, int64_t ld, uint8_t* q, int64_t ldq, uint8_t* s,

@@ -98,8 +98,26 @@ __device__ __forceinline__ int xcd_tile(int w, int nt) {
   return x * q + (x < r ? x : r) + s;
 }
 
+// LDS geometry of a tile configuration (elements of the LDS type)
+template <int OPK, int BM, int BN, int BK>
+struct TileLdsGeom {
+  // A rows 64 B apart modulo the 256-byte bank window (16-bit): the four rows
+  // and two m-halves of a transpose read hit all 64 banks once
+  static constexpr int PADA = 32;
+  static constexpr int PADB = (OPK == OPK_F32) ? 4 : 8;
+  static constexpr int SA = BM + PADA;  // As[k][m]
+  static constexpr int SB = BK + PADB;  // Bs[n][k]
+  static constexpr int A_ELEMS = BK * SA, B_ELEMS = BN * SB;  // per buffer
+  static constexpr size_t BYTES =
+      2 * (size_t)(A_ELEMS + B_ELEMS) * sizeof(typename OpTraits<OPK>::Lds);
+};
+
+// One workgroup's tile: `bid` is its tile index (remapped below), `bz` its
+// K slice; As0 / Bs0: two LDS buffers each of A and B slabs.
 template <int OPK, typename OutT, int BM, int BN, int WM, int WN, int BK, bool VEC, bool PRELOAD>
-__global__ void __launch_bounds__(256) gemm_mfma_kernel(GemmArgs g) {
+__device__ __forceinline__ void gemm_mfma_body(const GemmArgs& g, int bid, int bz, bool split,
+                                               typename OpTraits<OPK>::Lds* As0,
+                                               typename OpTraits<OPK>::Lds* Bs0) {
   using TR = OpTraits<OPK>;
   using S = typename TR::Store;
   using L = typename TR::Lds;
@@ -107,20 +125,16 @@ __global__ void __launch_bounds__(256) gemm_mfma_kernel(GemmArgs g) {
   constexpr int CH = TR::CH;
   constexpr int WTM = BM / WM, WTN = BN / WN;
   constexpr int FM = WTM / 32, FN = WTN / 32;
-  // A rows 64 B apart modulo the 256-byte bank window (16-bit): the four rows
-  // and two m-halves of a transpose read hit all 64 banks once
-  constexpr int PADA = 32;
-  constexpr int PADB = (OPK == OPK_F32) ? 4 : 8;
-  constexpr int SA = BM + PADA;  // As[k][m]
-  constexpr int SB = BK + PADB;  // Bs[n][k]
+  using GEO = TileLdsGeom<OPK, BM, BN, BK>;
+  constexpr int SA = GEO::SA, SB = GEO::SB;
   constexpr int TA = BK * BM / CH, TB = BN * BK / CH;  // 16-byte chunks per slab
   constexpr int NA = (TA + 255) / 256;
   constexpr int NB = (TB + 255) / 256;
   static_assert(TA % 64 == 0 && TB % 64 == 0, "slab chunks must fill whole waves");
   static_assert(FM >= 1 && FN >= 1, "wave tile below 32x32");
 
-  __shared__ __attribute__((aligned(16))) L As[2][BK * SA];
-  __shared__ __attribute__((aligned(16))) L Bs[2][BN * SB];
+  auto As = [&](int buf) { return As0 + buf * GEO::A_ELEMS; };
+  auto Bs = [&](int buf) { return Bs0 + buf * GEO::B_ELEMS; };
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -132,18 +146,17 @@ __global__ void __launch_bounds__(256) gemm_mfma_kernel(GemmArgs g) {
   // grouped raster: GM tile rows at a time, all their tile columns, so the
   // tiles sharing an A row block (and a B column block) run together on one
   // XCD and meet in its L2
-  const int tile = xcd_tile(blockIdx.x, nt);
+  const int tile = xcd_tile(bid, nt);
   constexpr int GM = 8;
   const int gsz = GM * g.tiles_n;
   const int grp = tile / gsz, first = grp * GM;
   const int rows = g.tiles_m - first < GM ? g.tiles_m - first : GM;
   const int tm = first + (tile % gsz) % rows, tn = (tile % gsz) / rows;
   const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
-  const int64_t kbeg = (int64_t)blockIdx.z * g.kchunk;
+  const int64_t kbeg = (int64_t)bz * g.kchunk;
   const int64_t kend = kbeg + g.kchunk < g.K ? kbeg + g.kchunk : g.K;
   const int64_t klen = kend > kbeg ? kend - kbeg : 0;
   const int nk = (int)((klen + BK - 1) / BK);
-  const bool split = gridDim.z > 1;
 
   const S* __restrict__ A = (const S*)g.A;
   const S* __restrict__ B = (const S*)g.B;
@@ -213,7 +226,7 @@ __global__ void __launch_bounds__(256) gemm_mfma_kernel(GemmArgs g) {
       int mq = c % (BM / CH), k = c / (BM / CH);
       Chunk v = ra[p];
       if constexpr (VEC) v = oka[p] ? v : (Chunk)0;
-      *(Chunk*)&As[buf][k * SA + mq * CH] = TR::cvt(v);
+      *(Chunk*)&As(buf)[k * SA + mq * CH] = TR::cvt(v);
     }
 #pragma unroll
     for (int p = 0; p < NB; ++p) {
@@ -222,7 +235,7 @@ __global__ void __launch_bounds__(256) gemm_mfma_kernel(GemmArgs g) {
       int kq = c % (BK / CH), n = c / (BK / CH);
       Chunk v = rb[p];
       if constexpr (VEC) v = okb[p] ? v : (Chunk)0;
-      *(Chunk*)&Bs[buf][n * SB + kq * CH] = TR::cvt(v);
+      *(Chunk*)&Bs(buf)[n * SB + kq * CH] = TR::cvt(v);
     }
   };
 
@@ -248,8 +261,8 @@ __global__ void __launch_bounds__(256) gemm_mfma_kernel(GemmArgs g) {
   }
 
   auto compute = [&](int buf) {
-    const L* as = As[buf];
-    const L* bs = Bs[buf];
+    const L* as = As(buf);
+    const L* bs = Bs(buf);
     if constexpr (OPK == OPK_F32) {
 #pragma unroll
       for (int ks = 0; ks < BK; ks += 2) {
@@ -326,7 +339,7 @@ __global__ void __launch_bounds__(256) gemm_mfma_kernel(GemmArgs g) {
         for (int q = 0; q < 16; ++q) {
           const int64_t n = nb + 8 * (q >> 2) + (q & 3);
           if (m < g.M && n < g.N)
-            g.W[((int64_t)blockIdx.z * g.N + n) * g.ldw + m] = g.alpha * acc[i][j][q];
+            g.W[((int64_t)bz * g.N + n) * g.ldw + m] = g.alpha * acc[i][j][q];
         }
       } else if (from_c || g.beta == 0.0f) {
 #pragma unroll
@@ -352,4 +365,35 @@ __global__ void __launch_bounds__(256) gemm_mfma_kernel(GemmArgs g) {
       }
     }
   }
+}
+
+template <int OPK, typename OutT, int BM, int BN, int WM, int WN, int BK, bool VEC, bool PRELOAD>
+__global__ void __launch_bounds__(256) gemm_mfma_kernel(GemmArgs g) {
+  using L = typename OpTraits<OPK>::Lds;
+  using GEO = TileLdsGeom<OPK, BM, BN, BK>;
+  __shared__ __attribute__((aligned(16))) L As[2 * GEO::A_ELEMS];
+  __shared__ __attribute__((aligned(16))) L Bs[2 * GEO::B_ELEMS];
+  gemm_mfma_body<OPK, OutT, BM, BN, WM, WN, BK, VEC, PRELOAD>(g, blockIdx.x, blockIdx.z,
+                                                              gridDim.z > 1, As, Bs);
+}
+
+// Two independent GEMMs in one launch (no split-K): workgroups [0, nt0) run
+// g0 with a 256x32 tile, the rest g1 with a 32x256 tile. The slice updates
+// of the LL inner factorization come in such pairs (the slice's columns,
+// M x 64, and its rows, 64 x N, with the same K); one launch lets them fill
+// the chip together instead of one after the other.
+template <int OPK, typename OutT, int BK, bool VEC>
+__global__ void __launch_bounds__(256) gemm_pair_kernel(GemmArgs g0, GemmArgs g1, int nt0) {
+  using L = typename OpTraits<OPK>::Lds;
+  using G0 = TileLdsGeom<OPK, 256, 32, BK>;
+  using G1 = TileLdsGeom<OPK, 32, 256, BK>;
+  constexpr int N0 = 2 * (G0::A_ELEMS + G0::B_ELEMS), N1 = 2 * (G1::A_ELEMS + G1::B_ELEMS);
+  __shared__ __attribute__((aligned(16))) L smem[N0 > N1 ? N0 : N1];  // one tile's buffers
+  const int b = blockIdx.x;
+  if (b < nt0)
+    gemm_mfma_body<OPK, OutT, 256, 32, 4, 1, BK, VEC, false>(g0, b, 0, false, smem,
+                                                             smem + 2 * G0::A_ELEMS);
+  else
+    gemm_mfma_body<OPK, OutT, 32, 256, 1, 4, BK, VEC, false>(g1, b - nt0, 0, false, smem,
+                                                             smem + 2 * G1::A_ELEMS);
 }
