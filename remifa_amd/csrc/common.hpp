@@ -207,6 +207,13 @@ int gemm_launch(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t m, in
                 float alpha, const void* a, int64_t lda, const void* b, int64_t ldb, float beta,
                 void* c, int64_t ldc);
 
+// C = alpha A B + beta Cin with the beta term read from another matrix
+// (fp16 or fp32, ld ldcin) and the result written to C (matrix-core
+// compute types FP32 / TC32 / BF16_TC32; cin = null is gemm_launch)
+int gemm_launch_cin(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t m, int64_t n,
+                    int64_t k, float alpha, const void* a, int64_t lda, const void* b, int64_t ldb,
+                    float beta, const void* cin, int cin_t, int64_t ldcin, void* c, int64_t ldc);
+
 // C0 -= A0 B0 (m0 x n0) and C1 -= A1 B1 (m1 x n1), both with inner size k:
 // one launch when the narrow tiles apply, else two gemm_launch calls
 int gemm_pair_launch(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t m0, int64_t n0,

@@ -325,14 +325,21 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
           // (the diagonal block is in both buffers, as in the reference's
           // d_l_tmp / d_u_tmp: L and the diagonal tiles are read from pl,
           // the U part right of each tile from pu)
-          RF_CHECK(convert_launch(h, W, REMIFA_R_32F, updm, in, blk, ldda, pl, ldpl));
-          RF_CHECK(convert_launch(h, W, REMIFA_R_32F, in, updm, blk, ldda, pu, ldpu));
+          if (c.OP == REMIFA_R_32F && ofs > 0 && !fp8) {
+            // block column and block row straight into the fp32 buffers:
+            // buffer = A (fp16, read by the epilogue) - L U (no separate
+            // fp16 -> fp32 copy of the block first)
+            RF_CHECK(gemm_launch_cin(h, c.O, operand_type(c.O), REMIFA_R_32F, updm, in, ofs, -1.0f,
+                                     la, lo, ua, lo, 1.0f, blk, W, ldda, pl, ldpl));
+            RF_CHECK(gemm_launch_cin(h, c.O, operand_type(c.O), REMIFA_R_32F, in, updm, ofs, -1.0f,
+                                     la, lo, ua, lo, 1.0f, blk, W, ldda, pu, ldpu));
+          } else {
+            RF_CHECK(convert_launch(h, W, REMIFA_R_32F, updm, in, blk, ldda, pl, ldpl));
+            RF_CHECK(convert_launch(h, W, REMIFA_R_32F, in, updm, blk, ldda, pu, ldpu));
+          }
           if (c.OP == REMIFA_R_32F && ofs > 0 && fp8) {
             RF_CHECK(gemm8(REMIFA_R_32F, updm, in, ofs, ofs, ofs, pl, ldpl));
             RF_CHECK(gemm8(REMIFA_R_32F, in, updm, ofs, ofs, ofs, pu, ldpu));
-          } else if (c.OP == REMIFA_R_32F && ofs > 0) {
-            RF_CHECK(gemm(x, c.O, REMIFA_R_32F, updm, in, ofs, la, lo, ua, lo, pl, ldpl));
-            RF_CHECK(gemm(x, c.O, REMIFA_R_32F, in, updm, ofs, la, lo, ua, lo, pu, ldpu));
           }
           // inner factorization in fp32; its fp16 mirror is A itself (the
           // reference passes d_a as the fp16 copy, src/lu_nopiv_ll.cxx:632),

@@ -42,15 +42,21 @@ template <typename OutT>
 __global__ void __launch_bounds__(256) splitk_reduce_kernel(int64_t M, int64_t N, int split,
                                                             const float* __restrict__ W,
                                                             int64_t ldw, float beta,
-                                                            OutT* __restrict__ C, int64_t ldc) {
+                                                            OutT* __restrict__ C, int64_t ldc,
+                                                            const void* Cin, int64_t ldcin,
+                                                            int cin_half) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= M) return;
   for (int64_t j = blockIdx.y; j < N; j += gridDim.y) {
     float s = 0.0f;
     for (int z = 0; z < split; ++z) s += W[(z * N + j) * ldw + i];
     OutT* cp = C + i + j * ldc;
-    float r = (beta != 0.0f) ? s + beta * (float)*cp : s;
-    *cp = (OutT)r;
+    float cv = 0.0f;
+    if (beta != 0.0f)
+      cv = Cin == nullptr ? (float)*cp
+                          : (cin_half ? (float)((const half_t*)Cin)[i + j * ldcin]
+                                      : ((const float*)Cin)[i + j * ldcin]);
+    *cp = (OutT)((beta != 0.0f) ? s + beta * cv : s);
   }
 }
 
@@ -393,10 +399,10 @@ int fp8_gemm_launch(remifa_handle_t h, int out_t, int64_t m, int64_t n, int64_t 
     dim3 rg((unsigned)ceil_div(m, 256), (unsigned)(n < 65535 ? n : 65535));
     if (out_t == REMIFA_R_32F)
       splitk_reduce_kernel<float><<<rg, 256, 0, h->stream>>>(m, n, split, g.W, g.ldw, beta,
-                                                             (float*)c, ldc);
+                                                             (float*)c, ldc, nullptr, 0, 0);
     else
       splitk_reduce_kernel<half_t><<<rg, 256, 0, h->stream>>>(m, n, split, g.W, g.ldw, beta,
-                                                              (half_t*)c, ldc);
+                                                              (half_t*)c, ldc, nullptr, 0, 0);
     if (hipGetLastError() != hipSuccess) st = REMIFA_ERR_HIP;
   }
   timer_end(h);
@@ -406,10 +412,20 @@ int fp8_gemm_launch(remifa_handle_t h, int out_t, int64_t m, int64_t n, int64_t 
 int gemm_launch(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t m, int64_t n, int64_t k,
                 float alpha, const void* a, int64_t lda, const void* b, int64_t ldb, float beta,
                 void* c, int64_t ldc) {
+  return gemm_launch_cin(h, ctype, in_t, out_t, m, n, k, alpha, a, lda, b, ldb, beta, nullptr, 0,
+                         0, c, ldc);
+}
+
+int gemm_launch_cin(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t m, int64_t n,
+                    int64_t k, float alpha, const void* a, int64_t lda, const void* b, int64_t ldb,
+                    float beta, const void* cin, int cin_t, int64_t ldcin, void* c, int64_t ldc) {
   if (m <= 0 || n <= 0) return 0;
+  if (cin && (ctype == REMIFA_FP16 || ctype == REMIFA_TC16 || ctype == REMIFA_FP8_TC32 ||
+              (cin_t != REMIFA_R_16F && cin_t != REMIFA_R_32F)))
+    return REMIFA_ERR_UNSUPPORTED;
   const double flops = 2.0 * (double)m * (double)n * (double)(k > 0 ? k : 0);
   if (k <= 0 || alpha == 0.0f) {
-    if (beta == 1.0f) return 0;
+    if (beta == 1.0f && !cin) return 0;
     // C = beta * C: run the GEMM with K = 0 (the epilogue scales)
   }
   int opk;
@@ -478,6 +494,9 @@ int gemm_launch(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t m, in
   g.A = a;
   g.B = b;
   g.C = c;
+  g.Cin = cin;
+  g.ldcin = ldcin;
+  g.cin_half = cin_t == REMIFA_R_16F;
   const size_t es = (opk == OPK_F32) ? 4 : 2;
   const int chunk = 16 / (int)es;
   // whole 16-byte chunks: aligned operands, M and K multiples of the chunk
@@ -547,10 +566,12 @@ int gemm_launch(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t m, in
     dim3 grid((unsigned)ceil_div(m, 256), (unsigned)(n < 65535 ? n : 65535));
     if (out_t == REMIFA_R_32F)
       splitk_reduce_kernel<float><<<grid, 256, 0, h->stream>>>(m, n, split, g.W, g.ldw, beta,
-                                                               (float*)c, ldc);
+                                                               (float*)c, ldc, g.Cin, g.ldcin,
+                                                               g.cin_half);
     else
       splitk_reduce_kernel<half_t><<<grid, 256, 0, h->stream>>>(m, n, split, g.W, g.ldw, beta,
-                                                                (half_t*)c, ldc);
+                                                                (half_t*)c, ldc, g.Cin, g.ldcin,
+                                                                g.cin_half);
     if (hipGetLastError() != hipSuccess) st = REMIFA_ERR_HIP;
   }
   timer_end(h);

@@ -52,7 +52,32 @@ struct GemmArgs {
   int64_t ldw;
   int64_t kchunk;  // K range per split (multiple of BK)
   int tiles_m, tiles_n;
+  // the beta term from another matrix (C = alpha A B + beta Cin), fp16 or
+  // fp32; null: Cin = C
+  const void* Cin;
+  int64_t ldcin;
+  int cin_half;
 };
+
+// 16 elements (m, n0 + 8 (q / 4) + q % 4) of the beta operand, as floats; the
+// source is chosen once, so the 16 loads of a branch are issued together
+template <typename OutT>
+__device__ __forceinline__ void c_in16(const GemmArgs& g, int64_t m, int64_t n0, float* cv) {
+  auto idx = [&](int q, int64_t ld) {
+    const int64_t n = n0 + 8 * (q >> 2) + (q & 3);
+    return (m < g.M && n < g.N) ? m + n * ld : 0;
+  };
+  if (g.Cin == nullptr) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) cv[q] = (float)((const OutT*)g.C)[idx(q, g.ldc)];
+  } else if (g.cin_half) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) cv[q] = (float)((const half_t*)g.Cin)[idx(q, g.ldcin)];
+  } else {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) cv[q] = ((const float*)g.Cin)[idx(q, g.ldcin)];
+  }
+}
 
 template <int OPK>
 struct OpTraits;
@@ -350,12 +375,7 @@ __device__ __forceinline__ void gemm_mfma_body(const GemmArgs& g, int bid, int b
       } else {
         // general alpha / beta: read this fragment of C, then write it
         float cv[16];
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-          const int64_t n = nb + 8 * (q >> 2) + (q & 3);
-          bool ok = m < g.M && n < g.N;
-          cv[q] = (float)Cr[ok ? m + n * g.ldc : 0];
-        }
+        c_in16<OutT>(g, m, nb, cv);
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
           const int64_t n = nb + 8 * (q >> 2) + (q & 3);
