@@ -192,7 +192,10 @@ __device__ __forceinline__ void gemm_mfma_body(const GemmArgs& g, int bid, int b
   // in flight while the matrix cores work
   bool oka[NA], okb[NB];
 
-  auto load_slab = [&](int kt) {
+  // FULL (a tag type): every slab of this tile is in range, so the loads
+  // need no address clamps and the stores no zero-selects
+  auto load_slab = [&](int kt, auto full_tag) {
+    constexpr bool FULL = decltype(full_tag)::value;
     const int64_t kk0 = kbeg + (int64_t)kt * BK;
 #pragma unroll
     for (int p = 0; p < NA; ++p) {
@@ -200,7 +203,9 @@ __device__ __forceinline__ void gemm_mfma_body(const GemmArgs& g, int bid, int b
       if (TA % 256 && c >= TA) break;
       int mq = c % (BM / CH), k = c / (BM / CH);
       int64_t gm = m0 + mq * CH, gk = kk0 + k;
-      if constexpr (VEC) {
+      if constexpr (VEC && FULL) {
+        ra[p] = *(const Chunk*)(A + gm + gk * g.lda);
+      } else if constexpr (VEC) {
         const bool ok = gm < g.M && gk < kend;
         ra[p] = *(const Chunk*)(A + (ok ? gm + gk * g.lda : 0));
         oka[p] = ok;
@@ -224,7 +229,9 @@ __device__ __forceinline__ void gemm_mfma_body(const GemmArgs& g, int bid, int b
       if (TB % 256 && c >= TB) break;
       int kq = c % (BK / CH), n = c / (BK / CH);
       int64_t gk = kk0 + kq * CH, gn = n0 + n;
-      if constexpr (VEC) {
+      if constexpr (VEC && FULL) {
+        rb[p] = *(const Chunk*)(B + gk + gn * g.ldb);
+      } else if constexpr (VEC) {
         const bool ok = gn < g.N && gk < kend;
         rb[p] = *(const Chunk*)(B + (ok ? gk + gn * g.ldb : 0));
         okb[p] = ok;
@@ -243,14 +250,15 @@ __device__ __forceinline__ void gemm_mfma_body(const GemmArgs& g, int bid, int b
       }
     }
   };
-  auto store_tiles = [&](int buf) {
+  auto store_tiles = [&](int buf, auto full_tag) {
+    constexpr bool FULL = decltype(full_tag)::value;
 #pragma unroll
     for (int p = 0; p < NA; ++p) {
       int c = tid + 256 * p;
       if (TA % 256 && c >= TA) break;
       int mq = c % (BM / CH), k = c / (BM / CH);
       Chunk v = ra[p];
-      if constexpr (VEC) v = oka[p] ? v : (Chunk)0;
+      if constexpr (VEC && !FULL) v = oka[p] ? v : (Chunk)0;
       *(Chunk*)&As(buf)[k * SA + mq * CH] = TR::cvt(v);
     }
 #pragma unroll
@@ -259,7 +267,7 @@ __device__ __forceinline__ void gemm_mfma_body(const GemmArgs& g, int bid, int b
       if (TB % 256 && c >= TB) break;
       int kq = c % (BK / CH), n = c / (BK / CH);
       Chunk v = rb[p];
-      if constexpr (VEC) v = okb[p] ? v : (Chunk)0;
+      if constexpr (VEC && !FULL) v = okb[p] ? v : (Chunk)0;
       *(Chunk*)&Bs(buf)[n * SB + kq * CH] = TR::cvt(v);
     }
   };
@@ -338,19 +346,26 @@ __device__ __forceinline__ void gemm_mfma_body(const GemmArgs& g, int bid, int b
     }
   };
 
-  if (nk > 0) {
-    load_slab(0);
-    store_tiles(0);
-  }
-  __syncthreads();
-  for (int kt = 0; kt < nk; ++kt) {
-    const int buf = kt & 1;
-    const bool more = kt + 1 < nk;
-    if (more) load_slab(kt + 1);
-    compute(buf);
-    if (more) store_tiles(buf ^ 1);
+  auto run = [&](auto full_tag) {
+    if (nk > 0) {
+      load_slab(0, full_tag);
+      store_tiles(0, full_tag);
+    }
     __syncthreads();
-  }
+    for (int kt = 0; kt < nk; ++kt) {
+      const int buf = kt & 1;
+      const bool more = kt + 1 < nk;
+      if (more) load_slab(kt + 1, full_tag);
+      compute(buf);
+      if (more) store_tiles(buf ^ 1, full_tag);
+      __syncthreads();
+    }
+  };
+  // interior tiles (the bulk of the LU's updates) take the unmasked loop
+  if (VEC && m0 + BM <= g.M && n0 + BN <= g.N && klen % BK == 0)
+    run(std::integral_constant<bool, true>{});
+  else
+    run(std::integral_constant<bool, false>{});
 
   // epilogue: lane holds C[m = .. + l32][n = .. + 8*(q/4) + 4*lh + q%4]
 #pragma unroll
