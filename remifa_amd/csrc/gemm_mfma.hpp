@@ -192,20 +192,49 @@ __device__ __forceinline__ void gemm_mfma_body(const GemmArgs& g, int bid, int b
   // in flight while the matrix cores work
   bool oka[NA], okb[NB];
 
+  // FULL path addressing: a uniform slab base (advanced by the scalar unit)
+  // plus per-lane 32-bit element offsets fixed for the whole loop
+  const S* const At = A + m0 + kbeg * g.lda;
+  const S* const Bt = B + kbeg + n0 * g.ldb;
+  uint32_t offa[NA], offb[NB];
+#pragma unroll
+  for (int p = 0; p < NA; ++p) {
+    const int c = tid + 256 * p;
+    offa[p] = (uint32_t)((c % (BM / CH)) * CH + (int64_t)(c / (BM / CH)) * g.lda);
+  }
+#pragma unroll
+  for (int p = 0; p < NB; ++p) {
+    const int c = tid + 256 * p;
+    offb[p] = (uint32_t)((c % (BK / CH)) * CH + (int64_t)(c / (BK / CH)) * g.ldb);
+  }
+
   // FULL (a tag type): every slab of this tile is in range, so the loads
   // need no address clamps and the stores no zero-selects
   auto load_slab = [&](int kt, auto full_tag) {
     constexpr bool FULL = decltype(full_tag)::value;
     const int64_t kk0 = kbeg + (int64_t)kt * BK;
+    if constexpr (VEC && FULL) {
+      const S* const as = At + (int64_t)kt * BK * g.lda;
+      const S* const bs = Bt + (int64_t)kt * BK;
+#pragma unroll
+      for (int p = 0; p < NA; ++p) {
+        if (TA % 256 && tid + 256 * p >= TA) break;
+        ra[p] = *(const Chunk*)(as + offa[p]);
+      }
+#pragma unroll
+      for (int p = 0; p < NB; ++p) {
+        if (TB % 256 && tid + 256 * p >= TB) break;
+        rb[p] = *(const Chunk*)(bs + offb[p]);
+      }
+      return;
+    }
 #pragma unroll
     for (int p = 0; p < NA; ++p) {
       int c = tid + 256 * p;
       if (TA % 256 && c >= TA) break;
       int mq = c % (BM / CH), k = c / (BM / CH);
       int64_t gm = m0 + mq * CH, gk = kk0 + k;
-      if constexpr (VEC && FULL) {
-        ra[p] = *(const Chunk*)(A + gm + gk * g.lda);
-      } else if constexpr (VEC) {
+      if constexpr (VEC) {
         const bool ok = gm < g.M && gk < kend;
         ra[p] = *(const Chunk*)(A + (ok ? gm + gk * g.lda : 0));
         oka[p] = ok;
@@ -229,9 +258,7 @@ __device__ __forceinline__ void gemm_mfma_body(const GemmArgs& g, int bid, int b
       if (TB % 256 && c >= TB) break;
       int kq = c % (BK / CH), n = c / (BK / CH);
       int64_t gk = kk0 + kq * CH, gn = n0 + n;
-      if constexpr (VEC && FULL) {
-        rb[p] = *(const Chunk*)(B + gk + gn * g.ldb);
-      } else if constexpr (VEC) {
+      if constexpr (VEC) {
         const bool ok = gn < g.N && gk < kend;
         rb[p] = *(const Chunk*)(B + (ok ? gk + gn * g.ldb : 0));
         okb[p] = ok;
@@ -362,7 +389,9 @@ __device__ __forceinline__ void gemm_mfma_body(const GemmArgs& g, int bid, int b
     }
   };
   // interior tiles (the bulk of the LU's updates) take the unmasked loop
-  if (VEC && m0 + BM <= g.M && n0 + BN <= g.N && klen % BK == 0)
+  // (when the lane offsets above fit in 32 bits)
+  if (VEC && m0 + BM <= g.M && n0 + BN <= g.N && klen % BK == 0 &&
+      (int64_t)BK * g.lda < (1LL << 31) && (int64_t)BN * g.ldb < (1LL << 31))
     run(std::integral_constant<bool, true>{});
   else
     run(std::integral_constant<bool, false>{});
