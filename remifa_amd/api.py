@@ -136,7 +136,7 @@ class Handle:
 
 
 # GEMM variant codes of the timer: opk << 8 | fp16-output << 4 | tile
-_GEMM_OPK = {0: "fp16", 1: "bf16", 2: "fp32"}
+_GEMM_OPK = {0: "fp16", 1: "bf16", 2: "fp32", 4: "bf16-stored"}
 _GEMM_TILE = {0: (256, 32, 4, 1), 1: (32, 256, 1, 4), 2: (256, 64, 4, 1), 3: (64, 256, 1, 4),
               4: (256, 256, 2, 2), 5: (128, 128, 2, 2), 6: (256, 128, 2, 2), 7: (128, 256, 2, 2)}
 

@@ -166,7 +166,9 @@ int remifa_gemmex(remifa_handle_t h, int ctype, int in_t, int out_t, int m, int 
                   int ldc) {
   if (!h) return -1;
   if (ctype < REMIFA_FP32 || ctype > REMIFA_FP8_TC32) return -2;
-  if (in_t != REMIFA_R_32F && in_t != REMIFA_R_16F) return -3;
+  if (in_t != REMIFA_R_32F && in_t != REMIFA_R_16F &&
+      !(in_t == REMIFA_R_16BF && ctype == REMIFA_BF16_TC32))
+    return -3;
   if (out_t != REMIFA_R_32F && out_t != REMIFA_R_16F) return -4;
   if (m < 0) return -5;
   if (n < 0) return -6;

@@ -188,6 +188,9 @@ struct Fp8Mirror {
 size_t lu_workspace_bytes(int algo, const remifa_lu_config_t& c, int64_t m, int64_t n, int64_t nb,
                           bool have_cpy) {
   size_t b = (subpanel_table_bytes() + 255) & ~(size_t)255;  // slice table
+  if (algo == REMIFA_LL && c.O == REMIFA_BF16_TC32 && c.W == REMIFA_R_16F &&
+      c.IP == REMIFA_R_32F)
+    b += ((size_t)m * n * 2 + 255) & ~(size_t)255;  // bf16 mirror of the factors
   if (c.O == REMIFA_FP8_TC32) {
     if (algo == REMIFA_LL)
       b += Fp8Mirror::bytes(m, n);
@@ -244,6 +247,11 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
     pl = ar.take((size_t)m * nb * 4);
     pu = ar.take((size_t)nb * m * 4);
   }
+  // BF16 outer updates with fp32 inner buffers: a bf16 mirror of the
+  // finished factors (each block rounded once) feeds the outer GEMMs
+  const bool bf16m = buffered && c.O == REMIFA_BF16_TC32;
+  void* b16 = bf16m ? ar.take((size_t)m * n * 2) : nullptr;
+  const int64_t ldb16 = m;
   Fp8Mirror f8;
   const bool fp8 = algo == REMIFA_LL && c.O == REMIFA_FP8_TC32;
   if (fp8) {
@@ -292,6 +300,11 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
         const void* ua2 = o_copy ? at(cpy, REMIFA_R_16F, 0, ofs + in, ldc)
                                  : at(d_a, W, 0, ofs + in, ldda);
         const int64_t lo = o_copy ? ldc : ldda;
+        // operands of the buffered outer update: A itself, or its bf16 mirror
+        const int oin = bf16m ? REMIFA_R_16BF : operand_type(c.O);
+        const void* lb = bf16m ? at(b16, REMIFA_R_16BF, ofs, 0, ldb16) : la;
+        const void* ub = bf16m ? at(b16, REMIFA_R_16BF, 0, ofs, ldb16) : ua;
+        const int64_t lbo = bf16m ? ldb16 : lo;
         if (!buffered) {
           if (ofs > 0 && fp8) {
             RF_CHECK(gemm8(W, updm, in, ofs, ofs, ofs, blk, ldda));
@@ -329,10 +342,10 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
             // block column and block row straight into the fp32 buffers:
             // buffer = A (fp16, read by the epilogue) - L U (no separate
             // fp16 -> fp32 copy of the block first)
-            RF_CHECK(gemm_launch_cin(h, c.O, operand_type(c.O), REMIFA_R_32F, updm, in, ofs, -1.0f,
-                                     la, lo, ua, lo, 1.0f, blk, W, ldda, pl, ldpl));
-            RF_CHECK(gemm_launch_cin(h, c.O, operand_type(c.O), REMIFA_R_32F, in, updm, ofs, -1.0f,
-                                     la, lo, ua, lo, 1.0f, blk, W, ldda, pu, ldpu));
+            RF_CHECK(gemm_launch_cin(h, c.O, oin, REMIFA_R_32F, updm, in, ofs, -1.0f, lb, lbo, ub,
+                                     lbo, 1.0f, blk, W, ldda, pl, ldpl));
+            RF_CHECK(gemm_launch_cin(h, c.O, oin, REMIFA_R_32F, in, updm, ofs, -1.0f, lb, lbo, ub,
+                                     lbo, 1.0f, blk, W, ldda, pu, ldpu));
           } else {
             RF_CHECK(convert_launch(h, W, REMIFA_R_32F, updm, in, blk, ldda, pl, ldpl));
             RF_CHECK(convert_launch(h, W, REMIFA_R_32F, in, updm, blk, ldda, pu, ldpu));
@@ -346,6 +359,17 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
           // so the rounded factors land in A as they are produced
           RF_CHECK(inner_ll(x, REMIFA_R_32F, updm, in, pl, ldpl, pu, ldpu, blk, ldda, true));
           if (fp8) RF_CHECK(mirror8(ofs, in));
+          if (bf16m) {
+            // round the finished block column (below the diagonal block) and
+            // block row (right of it) to bf16 once
+            const int64_t r0 = ofs + in;
+            if (m > r0)
+              RF_CHECK(convert_launch(h, W, REMIFA_R_16BF, m - r0, in, at(d_a, W, r0, ofs, ldda),
+                                      ldda, at(b16, REMIFA_R_16BF, r0, ofs, ldb16), ldb16));
+            if (n > r0)
+              RF_CHECK(convert_launch(h, W, REMIFA_R_16BF, in, n - r0, at(d_a, W, ofs, r0, ldda),
+                                      ldda, at(b16, REMIFA_R_16BF, ofs, r0, ldb16), ldb16));
+          }
         }
       }
     } else {

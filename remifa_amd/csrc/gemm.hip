@@ -210,6 +210,15 @@ int split_model(int64_t M, int64_t N, int64_t K, TileGeom tg, int num_cu, double
 }
 
 template <int OPK, typename OutT>
+int dispatch_basic(remifa_handle_t h, int tile, GemmArgs g, int split, bool vec) {
+  switch (tile) {
+    case T256x32: return launch_mfma<OPK, OutT, 256, 32, 4, 1>(h, g, split, vec);
+    case T32x256: return launch_mfma<OPK, OutT, 32, 256, 1, 4>(h, g, split, vec);
+    default: return launch_mfma<OPK, OutT, 128, 128, 2, 2>(h, g, split, vec);
+  }
+}
+
+template <int OPK, typename OutT>
 int dispatch_tiles(remifa_handle_t h, int tile, GemmArgs g, int split, bool vec) {
   switch (tile) {
     case T256x32: return launch_mfma<OPK, OutT, 256, 32, 4, 1>(h, g, split, vec);
@@ -432,7 +441,12 @@ int gemm_launch_cin(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t m
   switch (ctype) {
     case REMIFA_FP32: opk = OPK_F32; if (in_t != REMIFA_R_32F) return -4; break;
     case REMIFA_TC32: opk = OPK_F16; if (in_t != REMIFA_R_16F) return -4; break;
-    case REMIFA_BF16_TC32: opk = OPK_BF16; if (in_t != REMIFA_R_16F) return -4; break;
+    case REMIFA_BF16_TC32:
+      // fp16 storage rounded to bf16 while staging, or bf16 storage as is
+      if (in_t == REMIFA_R_16F) opk = OPK_BF16;
+      else if (in_t == REMIFA_R_16BF) opk = OPK_BF16S;
+      else return -4;
+      break;
     case REMIFA_FP8_TC32: {
       // quantize A by rows and B by columns into the free workspace, then
       // the scaled matrix-core GEMM (K padded to whole 32-blocks with zeros)
@@ -558,9 +572,14 @@ int gemm_launch_cin(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t m
   } else if (opk == OPK_F16) {
     st = out_t == REMIFA_R_32F ? dispatch_tiles<OPK_F16, float>(h, tile, g, split, vec)
                                : dispatch_tiles<OPK_F16, half_t>(h, tile, g, split, vec);
-  } else {
+  } else if (opk == OPK_BF16) {
     st = out_t == REMIFA_R_32F ? dispatch_tiles<OPK_BF16, float>(h, tile, g, split, vec)
                                : dispatch_tiles<OPK_BF16, half_t>(h, tile, g, split, vec);
+  } else {
+    // bf16 storage (the LU's bf16 mirror of its factors): the three basic tiles
+    const int t = (tile == T256x32 || tile == T32x256) ? tile : T128x128;
+    st = out_t == REMIFA_R_32F ? dispatch_basic<OPK_BF16S, float>(h, t, g, split, vec)
+                               : dispatch_basic<OPK_BF16S, half_t>(h, t, g, split, vec);
   }
   if (st == 0 && split > 1) {
     dim3 grid((unsigned)ceil_div(m, 256), (unsigned)(n < 65535 ? n : 65535));

@@ -39,7 +39,8 @@ __device__ __forceinline__ v4u16 ds_read_tr16(const uint16_t* p) {
   return __builtin_bit_cast(v4u16, r);
 }
 
-enum { OPK_F16 = 0, OPK_BF16 = 1, OPK_F32 = 2 };
+// operand kinds: fp16, bf16 rounded from fp16 storage, fp32, bf16 storage
+enum { OPK_F16 = 0, OPK_BF16 = 1, OPK_F32 = 2, OPK_BF16S = 4 };
 
 struct GemmArgs {
   int64_t M, N, K;
@@ -105,6 +106,14 @@ struct OpTraits<OPK_BF16> {
     }
     return r;
   }
+};
+template <>
+struct OpTraits<OPK_BF16S> {
+  using Store = uint16_t;  // bf16 bits in HBM
+  using Lds = uint16_t;
+  static constexpr int CH = 8;
+  using Chunk = u16x8;
+  static __device__ __forceinline__ Chunk cvt(Chunk c) { return c; }
 };
 template <>
 struct OpTraits<OPK_F32> {

@@ -165,3 +165,23 @@ def test_bad_arguments_raise(handle, gpu):
     with pytest.raises(rf.RemifaError) as e:
         rf.lu_nopiv_ll_f32_fp32xtc32(handle, 8, 8, 4, a, 8, info, None)
     assert e.value.status == -8
+
+
+@pytest.mark.parametrize("OP,n,nb,ib", [("f32", 320, 64, 8), ("f32", 500, 96, 32), ("f16", 256, 64, 8)])
+def test_ll_bf16_outer_updates_match_oracle(handle, gpu, OP, n, nb, ib):
+    """LL with BF16_TC32 outer updates: fp32 inner buffers (outer operands
+    from the LU's bf16 mirror of its factors) and in place (operands rounded
+    while staged), against the oracle's restatement."""
+    import torch
+    rf = _rf()
+    a32 = orc.gen_diagdom_alpha(n, 23, 1.2)
+    a16 = orc.f32_to_f16_bits(a32)
+    op_rf, op_or = (rf.R_32F, orc.F32) if OP == "f32" else (rf.R_16F, orc.F16)
+    cfg = rf.api.make_config(W=rf.R_16F, I=rf.TC32, O=rf.BF16_TC32, OP=op_rf, ib=ib)
+    da = to_dev(a16, gpu, ld=n + 3)
+    assert rf.lu_nopiv(handle, da, nb, algo=rf.LL, cfg=cfg) == 0
+    torch.cuda.synchronize()
+    g = orc.f16_bits_to_f32(to_np(da))
+    o = orc.f16_bits_to_f32(orc.lu_nopiv("ll", orc.cfg(W=orc.F16, I=orc.TC32, O=orc.BF16TC32,
+                                                        OP=op_or, ib=ib), a16, nb))
+    _check(g, o, orc.lu_f64(orc.round_to(orc.F16, a32)), orc.F16)
