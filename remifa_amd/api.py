@@ -148,6 +148,8 @@ def gemm_variant_name(v: int) -> str:
     if opk == 3:
         return (f"gemm_fp8_kernel<{'_Float16' if out16 else 'float'}> "
                 f"(scaled e4m3 operands, 128x128 tile)")
+    if tile == 15:
+        return f"splitk_reduce_kernel<{'_Float16' if out16 else 'float'}> (split-K partial sums)"
     if tile == 8:
         return (f"gemm_pair_kernel<{opk}, {'_Float16' if out16 else 'float'}> "
                 f"({_GEMM_OPK.get(opk, '?')} operands, 256x32 + 32x256 tiles)")

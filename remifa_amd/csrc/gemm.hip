@@ -404,7 +404,10 @@ int fp8_gemm_launch(remifa_handle_t h, int out_t, int64_t m, int64_t n, int64_t 
   else
     gemm_fp8_kernel<half_t><<<grid, 256, 0, h->stream>>>(g);
   int st = hipGetLastError() == hipSuccess ? 0 : REMIFA_ERR_HIP;
+  timer_end(h);
   if (st == 0 && split > 1) {
+    // the reduction is timed as its own variant (tile code 15)
+    timer_begin(h, KC_GEMM, 0.0, (3 << 8) | ((out_t == REMIFA_R_16F) << 4) | 15);
     dim3 rg((unsigned)ceil_div(m, 256), (unsigned)(n < 65535 ? n : 65535));
     if (out_t == REMIFA_R_32F)
       splitk_reduce_kernel<float><<<rg, 256, 0, h->stream>>>(m, n, split, g.W, g.ldw, beta,
@@ -413,8 +416,8 @@ int fp8_gemm_launch(remifa_handle_t h, int out_t, int64_t m, int64_t n, int64_t 
       splitk_reduce_kernel<half_t><<<rg, 256, 0, h->stream>>>(m, n, split, g.W, g.ldw, beta,
                                                               (half_t*)c, ldc, nullptr, 0, 0);
     if (hipGetLastError() != hipSuccess) st = REMIFA_ERR_HIP;
+    timer_end(h);
   }
-  timer_end(h);
   return st;
 }
 
@@ -581,7 +584,10 @@ int gemm_launch_cin(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t m
     st = out_t == REMIFA_R_32F ? dispatch_basic<OPK_BF16S, float>(h, t, g, split, vec)
                                : dispatch_basic<OPK_BF16S, half_t>(h, t, g, split, vec);
   }
+  timer_end(h);
   if (st == 0 && split > 1) {
+    // the reduction is timed as its own variant (tile code 15)
+    timer_begin(h, KC_GEMM, 0.0, (opk << 8) | ((out_t == REMIFA_R_16F) << 4) | 15);
     dim3 grid((unsigned)ceil_div(m, 256), (unsigned)(n < 65535 ? n : 65535));
     if (out_t == REMIFA_R_32F)
       splitk_reduce_kernel<float><<<grid, 256, 0, h->stream>>>(m, n, split, g.W, g.ldw, beta,
@@ -592,8 +598,8 @@ int gemm_launch_cin(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t m
                                                                 (half_t*)c, ldc, g.Cin, g.ldcin,
                                                                 g.cin_half);
     if (hipGetLastError() != hipSuccess) st = REMIFA_ERR_HIP;
+    timer_end(h);
   }
-  timer_end(h);
   return st;
 }
 
