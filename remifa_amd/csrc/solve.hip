@@ -7,13 +7,15 @@
 // per block row instead of 128 dependent steps. Then one launch per triangle:
 // workgroups take 128-row block rows in dependency order from an atomic
 // ticket, so every block a workgroup waits for belongs to a workgroup that is
-// already running (no deadlock whatever the residency). Blocks finish in
-// order, so one published-block counter (agent-scope release/acquire) tells
-// a workgroup how many solution blocks it may read; block row i accumulates
-// B_i - sum_j T_ij X_j over them, multiplies by its inverted diagonal block
-// and publishes X_i. Every wait is bounded: on timeout the kernel records an
-// error and exits, so the grid always drains. The factors are read once
-// (HBM-bound).
+// already running (no deadlock whatever the residency). Block row i
+// accumulates B_i - sum_j T_ij X_j, multiplies by its inverted diagonal block
+// and publishes X_i as tagged 64-bit words (value, 1) written with
+// agent-scope atomic stores into a tag buffer cleared before the launch: a
+// reader retries until every word it needs carries the tag, so a value and
+// its "published" mark arrive together (one memory round trip per hop
+// instead of store, wait, counter, poll, load). Every wait is bounded: on
+// timeout the kernel records an error and exits, so the grid always drains.
+// The factors are read once (HBM-bound).
 //
 // Residual R = B - A X in FP64: 2-D grid over (256-row block, 512-column
 // chunk), fp64 fma per element, per-chunk partials summed in chunk order by a
@@ -30,7 +32,7 @@ constexpr int NBS = 128;     // block size of the triangular solves
 constexpr int TT = 4 * NBS;  // threads of a triangular-solve workgroup
 constexpr int QC = NBS / 4;  // columns of a block per lane (column quarter)
 constexpr int RC = 16;   // right-hand sides per launch
-constexpr long long kSpinLimit = 1LL << 28;
+constexpr long long kSpinLimit = 1LL << 22;  // retries of a wait (each a memory round trip)
 
 template <typename LT>
 __device__ __forceinline__ float ldf(const LT* p) {
@@ -141,17 +143,20 @@ __global__ void __launch_bounds__(NBS) diag_inv_kernel(int64_t n, const LT* __re
 // (row, cq) owns row `row` of the block and the column quarter cq (32
 // columns) of every 128 x 128 block it multiplies; the four partial sums are
 // added in LDS in a fixed order. NR: right-hand sides per launch (1 or 16).
-// sync[0] = ticket, sync[1] = error flag, sync[2] = blocks published.
+// sync[0] = ticket, sync[1] = error flag; tags[r + k n] = (value, 1) once
+// entry r of solution column k is published (zeroed before the launch).
 template <typename LT, bool LOWER, bool TRANS, int NR>
 __global__ void __launch_bounds__(TT) trsm_kernel(int64_t n, int nrhs, const LT* __restrict__ lu,
                                                    int64_t ld, float* __restrict__ b, int64_t ldb,
                                                    const float* __restrict__ inv,
-                                                   int* __restrict__ sync, int nblk) {
+                                                   int* __restrict__ sync,
+                                                   unsigned long long* __restrict__ tags,
+                                                   int nblk) {
   constexpr int CH = NR == 1 ? 2 : 1;  // solution blocks staged in LDS at a time
   constexpr int NP = NR == 1 ? 1 : NR + 1;
   __shared__ float Xs[CH * NBS][NR];
   __shared__ float Red[4][NBS][NP];
-  __shared__ int s_ticket, s_avail;
+  __shared__ int s_ticket, s_stop[2];
   const int t = threadIdx.x;
   const int row = t % NBS, cq = t / NBS;
   if (t == 0)
@@ -178,7 +183,6 @@ __global__ void __launch_bounds__(TT) trsm_kernel(int64_t n, int nrhs, const LT*
 
   const int nsteps = tk;  // blocks before this one in dependency order
   const int64_t grc = gr < n ? gr : n - 1;  // in-bounds row for the loads
-  int avail = 0;                            // blocks known to be published
   for (int s0 = 0; s0 < nsteps; s0 += CH) {
     const int cnt = nsteps - s0 < CH ? nsteps - s0 : CH;
     // this lane's entries of the chunk's blocks of T: issued before the wait
@@ -197,40 +201,38 @@ __global__ void __launch_bounds__(TT) trsm_kernel(int64_t n, int nrhs, const LT*
         tv[q][c] = (gr < n && c0 + c < n) ? v : 0.0f;
       }
     }
-    // wait until blocks s0 .. s0 + cnt - 1 (in dependency order) are published
-    if (avail < s0 + cnt) {
-      if (t == 0) {
-        long long spins = 0;
-        int a = __hip_atomic_load(&sync[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        while (a < s0 + cnt) {
-          if (++spins > kSpinLimit ||
-              __hip_atomic_load(&sync[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-            __hip_atomic_store(&sync[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            a = nblk;  // give up: the results are discarded by the host
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-          a = __hip_atomic_load(&sync[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // stage the chunk's solution blocks: agent-scope atomic loads of the
+    // tagged words, retried until every one carries the published tag
+    // (uniform decisions: a barrier vote and a double-buffered stop flag)
+    for (long long tries = 0;; ++tries) {
+      int bad = 0;
+      for (int e = t; e < cnt * NBS * NR; e += TT) {
+        const int rr = e % (cnt * NBS), k = e / (cnt * NBS);
+        const int q = rr / NBS, c = rr % NBS;
+        const int bj = LOWER ? s0 + q : nblk - 1 - (s0 + q);
+        const int64_t C = (int64_t)bj * NBS + c;
+        float v = 0.0f;
+        if (C < n && k < nrhs) {
+          const unsigned long long w = __hip_atomic_load(&tags[C + (int64_t)k * n],
+                                                         __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+          bad |= (unsigned)(w >> 32) != 1u;
+          v = __uint_as_float((unsigned)w);
         }
-        s_avail = a;
+        Xs[rr][k] = v;
       }
-      __syncthreads();
-      avail = s_avail;
+      const int slot = (int)(tries & 1);
+      if (t == 0)
+        s_stop[slot] = tries > kSpinLimit ||
+                       __hip_atomic_load(&sync[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+      if (!__syncthreads_or(bad)) break;
+      if (s_stop[slot]) {
+        // give up (the host reports the error and discards the results)
+        if (t == 0) __hip_atomic_store(&sync[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
     }
-    // stage the published solution blocks. They were written during this
-    // launch by other workgroups with agent-scope atomic stores; agent-scope
-    // atomic loads read them at the device coherence point (no cache
-    // invalidation needed).
-    for (int e = t; e < cnt * NBS * NR; e += TT) {
-      const int rr = e % (cnt * NBS), k = e / (cnt * NBS);
-      const int q = rr / NBS, c = rr % NBS;
-      const int bj = LOWER ? s0 + q : nblk - 1 - (s0 + q);
-      const int64_t C = (int64_t)bj * NBS + c;
-      Xs[rr][k] = (C < n && k < nrhs)
-                      ? __hip_atomic_load(&b[C + k * ldb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                      : 0.0f;
-    }
-    __syncthreads();
 #pragma unroll
     for (int q = 0; q < CH; ++q) {
       if (q < cnt) {
@@ -269,17 +271,13 @@ __global__ void __launch_bounds__(TT) trsm_kernel(int64_t n, int nrhs, const LT*
   __syncthreads();
   for (int k = cq; k < NR; k += 4) {
     const float xv = ((Red[0][row][k] + Red[1][row][k]) + Red[2][row][k]) + Red[3][row][k];
-    if (gr < n && k < nrhs)
-      __hip_atomic_store(&b[gr + k * ldb], xv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (gr < n && k < nrhs) {
+      b[gr + k * ldb] = xv;  // the caller's copy
+      __hip_atomic_store(&tags[gr + (int64_t)k * n],
+                         (1ull << 32) | (unsigned long long)__float_as_uint(xv), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
-  // the solution went out with agent-scope atomic stores, which leave no
-  // dirty cache lines to write back: waiting for their completion
-  // (s_waitcnt vmcnt(0)) orders them before the count below
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_s_waitcnt(0x0F70);
-  __syncthreads();
-  // blocks finish in dependency order: this one is number tk + 1
-  if (t == 0) __hip_atomic_store(&sync[2], tk + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void fill_int_kernel(int* p, int n, int v) {
@@ -486,17 +484,24 @@ __global__ void maxreduce_kernel(int64_t n, const double* __restrict__ v, double
   if (threadIdx.x == 0) *out = red[0];
 }
 
+__global__ void zero_u64_kernel(unsigned long long* p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 template <typename LT, bool LOWER, bool TRANS>
 int launch_trsm(remifa_handle_t h, int64_t n, int nrhs, const void* lu, int64_t ld, float* b,
-                int64_t ldb, const float* inv, int* sync) {
+                int64_t ldb, const float* inv, int* sync, unsigned long long* tags) {
   const int nblk = (int)ceil_div(n, NBS);
-  fill_int_kernel<<<1, 256, 0, h->stream>>>(sync, 3, 0);
+  fill_int_kernel<<<1, 256, 0, h->stream>>>(sync, 1, 0);  // the ticket
+  const int64_t nt = n * (int64_t)nrhs;
+  zero_u64_kernel<<<(unsigned)ceil_div(nt, 256), 256, 0, h->stream>>>(tags, nt);
   if (nrhs == 1)
     trsm_kernel<LT, LOWER, TRANS, 1>
-        <<<nblk, TT, 0, h->stream>>>(n, nrhs, (const LT*)lu, ld, b, ldb, inv, sync, nblk);
+        <<<nblk, TT, 0, h->stream>>>(n, nrhs, (const LT*)lu, ld, b, ldb, inv, sync, tags, nblk);
   else
     trsm_kernel<LT, LOWER, TRANS, RC>
-        <<<nblk, TT, 0, h->stream>>>(n, nrhs, (const LT*)lu, ld, b, ldb, inv, sync, nblk);
+        <<<nblk, TT, 0, h->stream>>>(n, nrhs, (const LT*)lu, ld, b, ldb, inv, sync, tags, nblk);
   RF_HIP(hipGetLastError());
   return 0;
 }
@@ -504,17 +509,18 @@ int launch_trsm(remifa_handle_t h, int64_t n, int nrhs, const void* lu, int64_t 
 template <typename LT>
 int getrs_typed(remifa_handle_t h, int trans, int64_t n, int64_t nrhs, const void* lu,
                 int64_t ld, float* b, int64_t ldb, const float* inv_l, const float* inv_u,
-                int* sync) {
+                int* sync, unsigned long long* tags) {
+  fill_int_kernel<<<1, 256, 0, h->stream>>>(sync, 2, 0);  // ticket and error flag
   for (int64_t k0 = 0; k0 < nrhs; k0 += RC) {
     int nr = (int)std::min<int64_t>(RC, nrhs - k0);
     float* bk = b + k0 * ldb;
     if (!trans) {
-      RF_CHECK((launch_trsm<LT, true, false>(h, n, nr, lu, ld, bk, ldb, inv_l, sync)));
-      RF_CHECK((launch_trsm<LT, false, false>(h, n, nr, lu, ld, bk, ldb, inv_u, sync)));
+      RF_CHECK((launch_trsm<LT, true, false>(h, n, nr, lu, ld, bk, ldb, inv_l, sync, tags)));
+      RF_CHECK((launch_trsm<LT, false, false>(h, n, nr, lu, ld, bk, ldb, inv_u, sync, tags)));
     } else {
       // A^T = U^T L^T: U^T (lower) then L^T (upper, unit)
-      RF_CHECK((launch_trsm<LT, true, true>(h, n, nr, lu, ld, bk, ldb, inv_u, sync)));
-      RF_CHECK((launch_trsm<LT, false, true>(h, n, nr, lu, ld, bk, ldb, inv_l, sync)));
+      RF_CHECK((launch_trsm<LT, true, true>(h, n, nr, lu, ld, bk, ldb, inv_u, sync, tags)));
+      RF_CHECK((launch_trsm<LT, false, true>(h, n, nr, lu, ld, bk, ldb, inv_l, sync, tags)));
     }
   }
   return 0;
@@ -528,16 +534,19 @@ struct TrsmWs {
   int* sync;
   float* inv_l;
   float* inv_u;
+  unsigned long long* tags;  // n x RC published-solution words
 };
 
 static size_t trsm_ws_bytes(int64_t n) {
-  return 256 + 2 * sizeof(float) * NBS * NBS * (size_t)ceil_div(n, NBS);
+  return 256 + 2 * sizeof(float) * NBS * NBS * (size_t)ceil_div(n, NBS) +
+         sizeof(unsigned long long) * (size_t)n * RC;
 }
 
 static TrsmWs trsm_ws(void* p, int64_t n) {
   char* c = (char*)p;
   const size_t inv = sizeof(float) * NBS * NBS * (size_t)ceil_div(n, NBS);
-  return TrsmWs{(int*)c, (float*)(c + 256), (float*)(c + 256 + inv)};
+  return TrsmWs{(int*)c, (float*)(c + 256), (float*)(c + 256 + inv),
+                (unsigned long long*)(c + 256 + 2 * inv)};
 }
 
 // invert the diagonal blocks of the stored factors (once per factorization)
@@ -565,8 +574,10 @@ int getrs_launch(remifa_handle_t h, int lu_t, int trans, int64_t n, int64_t nrhs
   const double es = lu_t == REMIFA_R_16F ? 2.0 : 4.0;
   timer_begin(h, KC_TRSM, (double)n * (double)n * es * (double)ceil_div(nrhs, RC), 0);
   int st = lu_t == REMIFA_R_16F
-               ? getrs_typed<half_t>(h, trans, n, nrhs, lu, ld, b, ldb, w.inv_l, w.inv_u, w.sync)
-               : getrs_typed<float>(h, trans, n, nrhs, lu, ld, b, ldb, w.inv_l, w.inv_u, w.sync);
+               ? getrs_typed<half_t>(h, trans, n, nrhs, lu, ld, b, ldb, w.inv_l, w.inv_u, w.sync,
+                                     w.tags)
+               : getrs_typed<float>(h, trans, n, nrhs, lu, ld, b, ldb, w.inv_l, w.inv_u, w.sync,
+                                    w.tags);
   timer_end(h);
   return st;
 }
