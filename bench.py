@@ -58,6 +58,10 @@ def parse():
                         "traffic, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     p.add_argument("--update", choices=["fp16", "bf16", "fp8"], default="fp16",
                    help="outer-update operands of the timed factorization")
+    p.add_argument("--no-lookahead", action="store_true",
+                   help="LL: factor each outer block after its whole update (one stream)")
+    p.add_argument("--reserve-cus", type=int, default=0,
+                   help="LL lookahead: CUs masked off the second stream")
     p.add_argument("--no-variants", action="store_true",
                    help="skip the BF16 / FP8 update variants and the nrhs=16 refinement")
     p.add_argument("--workload", choices=["lu", "batched", "autograd"], default="lu",
@@ -277,6 +281,7 @@ def main():
         del src, dst
     n, nb = a.n, a.nb
     h = rf.Handle()
+    h.set_lookahead(not a.no_lookahead, a.reserve_cus)
     s = h.stream
     A = rf.colmajor_empty(n, n, device=dev)
     rf.gen_diagdom(h, A, seed=a.seed, alpha=a.alpha)
@@ -363,11 +368,16 @@ def main():
     algo_bytes = None
     if algo == rf.LL:
         tot, cnt = 0, 0
+        # (with the lookahead block k's update is two launches of each shape:
+        # K = ofs - nb on the second stream, then K = nb for block k-1)
+        la = not a.no_lookahead and a.update != "fp8"
         for k in range(1, (n + nb - 1) // nb):
             ofs = k * nb
-            for (M, N, K) in ((n - ofs, min(nb, n - ofs), ofs), (min(nb, n - ofs), n - ofs, ofs)):
-                tot += 2 * (M * K + K * N) + 8 * M * N
-                cnt += 1
+            ks = ([ofs - nb] if k >= 2 else []) + [nb] if la else [ofs]
+            for K in ks:
+                for (M, N) in ((n - ofs, min(nb, n - ofs)), (min(nb, n - ofs), n - ofs)):
+                    tot += 2 * (M * K + K * N) + 8 * M * N
+                    cnt += 1
         algo_bytes = round(tot / cnt) if cnt else None
 
     flops = 2.0 * n**3 / 3.0
@@ -393,7 +403,8 @@ def main():
                                f"{ {'fp16': 'TC32', 'bf16': 'BF16_TC32', 'fp8': 'FP8_TC32'}[a.update]} "
                                f"MFMA outer updates, + FP64-residual IR to 1e-14, nrhs={a.nrhs}",
                    "n": n, "nb": nb, "ib": a.ib, "algo": a.algo, "alpha": a.alpha,
-                   "nrhs": a.nrhs, "seed": a.seed,
+                   "nrhs": a.nrhs, "seed": a.seed, "lookahead": algo == rf.LL and not a.no_lookahead and a.update != "fp8",
+                   "lookahead_reserved_cus": a.reserve_cus,
                    "precision": (f"W=f16 I=TC32 O={ {'fp16': 'TC32', 'bf16': 'BF16_TC32', 'fp8': 'FP8_TC32'}[a.update]}"
                                  + (" OP=IP=OF=IF=f32" if algo == rf.LL else ""))},
         "factor_ms": round(t_fac, 3),

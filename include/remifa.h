@@ -101,6 +101,18 @@ int remifa_set_gemm_tile(remifa_handle_t handle, int tile);
  * GEMM's M and N decides. Written by remifa_amd/autotune.py. No reference
  * counterpart. */
 int remifa_set_gemm_rules(remifa_handle_t handle, int count, const int64_t *rules);
+/* Left-looking lookahead (1 = on, the default; 0 = off). With fp32 inner
+ * buffers and TC32 / BF16_TC32 outer updates, the update of outer block
+ * k+1 by the blocks before k runs on a second stream of the handle while
+ * block k is factored; block k's own contribution follows on the handle's
+ * stream. Both streams are joined before remifa_lu_nopiv returns.
+ * reserve_cus (a multiple of 8, at most half the CUs; default 0): CUs
+ * masked off the second stream, spread evenly over the XCDs, so that the
+ * factorization of block k always finds free CUs (on MI355X any mask made
+ * the n = 32768 factorization slower, 16 CUs: 1.6x). No reference
+ * counterpart (src/lu_nopiv_ll.hxx:208-562 runs the block updates and
+ * factorizations in sequence on one stream). */
+int remifa_set_lookahead(remifa_handle_t handle, int on, int reserve_cus);
 const char *remifa_last_error(void);
 
 /* Per-kernel timing with hipEvents on the handle's stream. While enabled,

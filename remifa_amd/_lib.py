@@ -49,6 +49,7 @@ SIGNATURES = {
     "remifa_lu_workspace_size": [_vp, _i, C.POINTER(LuConfig), _i64, _i64, _i64, C.POINTER(_sz)],
     "remifa_set_tuning": [_vp, _i, _i],
     "remifa_set_gemm_tile": [_vp, _i],
+    "remifa_set_lookahead": [_vp, _i, _i],
     "remifa_last_error": [],
     "remifa_timing_enable": [_vp, _i],
     "remifa_timing_query": [_vp, _i, _pd, _pi64, _pd],

@@ -107,6 +107,12 @@ class Handle:
     def set_gemm_tile(self, tile: int = -1):
         check(lib().remifa_set_gemm_tile(self._h, tile), "remifa_set_gemm_tile")
 
+    def set_lookahead(self, on: bool = True, reserve_cus: int = 0):
+        """LL lookahead: the next outer block's update on a second stream,
+        masked off `reserve_cus` CUs (0 = no mask)"""
+        check(lib().remifa_set_lookahead(self._h, int(bool(on)), reserve_cus),
+              "remifa_set_lookahead")
+
     # per-kernel timing -------------------------------------------------
     def timing(self, enable: bool = True):
         check(lib().remifa_timing_enable(self._h, int(enable)), "remifa_timing_enable")

@@ -74,6 +74,13 @@ struct remifa_handle_st {
   int panel_width = 0;
   int split_k = 0;
   int gemm_tile = -1;  // -1: chosen per shape
+  // LL lookahead: the next outer block's update runs on a second stream
+  // while the current block is factored (1 = on, 0 = off)
+  int lookahead = 1;
+  int side_reserve = 0;        // CUs the side stream leaves to the handle's stream
+  int side_reserve_made = -1;  // ... as created
+  hipStream_t side = nullptr;  // created on first use, default priority
+  hipEvent_t ev_main = nullptr, ev_side = nullptr;
   // autotuned GEMM rules, first match wins: {opk (-1 any), m_lo, m_hi, n_lo,
   // n_hi, tile (-1 heuristic), split (0 heuristic)}, bounds inclusive
   std::vector<std::array<int64_t, 7>> gemm_rules;

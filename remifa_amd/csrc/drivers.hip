@@ -185,6 +185,46 @@ struct Fp8Mirror {
   }
 };
 
+// LL lookahead layout: fp16 storage, fp32 inner buffers and panels, 16-bit
+// outer operands (TC32 from A itself or BF16_TC32 from its bf16 mirror),
+// at least two outer blocks
+bool lookahead_layout(int algo, const remifa_lu_config_t& c, int64_t n, int64_t nb) {
+  return algo == REMIFA_LL && c.W == REMIFA_R_16F && c.IP == REMIFA_R_32F &&
+         c.OP == REMIFA_R_32F && c.O != REMIFA_FP8_TC32 && n > nb;
+}
+
+// The lookahead's second stream (default priority), masked to leave
+// h->side_reserve CUs to the handle's stream. The mask bits i = x + 8 y
+// reserved are y = 4 ((x + t) % 8), t < reserve / 8: reserve / 8 CUs of
+// every class i % 8 and of every run of 32 bits, so every XCD keeps the
+// same share whether the driver numbers CUs XCD by XCD or round-robin.
+int side_stream(remifa_handle_t h) {
+  if (h->side && h->side_reserve_made == h->side_reserve) return 0;
+  if (h->side) {
+    RF_HIP(hipStreamSynchronize(h->side));
+    RF_HIP(hipStreamDestroy(h->side));
+    h->side = nullptr;
+  }
+  if (!h->ev_main) RF_HIP(hipEventCreateWithFlags(&h->ev_main, hipEventDisableTiming));
+  if (!h->ev_side) RF_HIP(hipEventCreateWithFlags(&h->ev_side, hipEventDisableTiming));
+  const int ncu = h->num_cu;
+  const int r = h->side_reserve;
+  if (r == 0 || ncu != 256) {
+    RF_HIP(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
+  } else {
+    uint32_t mask[8];
+    for (auto& w : mask) w = 0xFFFFFFFFu;
+    for (int x = 0; x < 8; ++x)
+      for (int t = 0; t < r / 8; ++t) {
+        const int i = x + 8 * 4 * ((x + t) % 8);
+        mask[i / 32] &= ~(1u << (i % 32));
+      }
+    RF_HIP(hipExtStreamCreateWithCUMask(&h->side, 8, mask));
+  }
+  h->side_reserve_made = r;
+  return 0;
+}
+
 size_t lu_workspace_bytes(int algo, const remifa_lu_config_t& c, int64_t m, int64_t n, int64_t nb,
                           bool have_cpy) {
   size_t b = (subpanel_table_bytes() + 255) & ~(size_t)255;  // slice table
@@ -202,6 +242,12 @@ size_t lu_workspace_bytes(int algo, const remifa_lu_config_t& c, int64_t m, int6
   if (algo == REMIFA_LL && c.W == REMIFA_R_16F && c.IP == REMIFA_R_32F) {
     b += ((size_t)m * nb * 4 + 255) & ~(size_t)255;  // block column buffer
     b += ((size_t)nb * m * 4 + 255) & ~(size_t)255;  // block row buffer
+  }
+  if (lookahead_layout(algo, c, n, nb)) {
+    // the next block's buffers and the side stream's split-K region
+    b += ((size_t)m * nb * 4 + 255) & ~(size_t)255;
+    b += ((size_t)nb * m * 4 + 255) & ~(size_t)255;
+    b += gemm_reserve(m, nb);
   }
   return b + gemm_reserve(m, nb);
 }
@@ -247,6 +293,18 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
     pl = ar.take((size_t)m * nb * 4);
     pu = ar.take((size_t)nb * m * 4);
   }
+  // lookahead: block kk is factored in buffers plb/pub[kk % 2] while the
+  // side stream prepares block kk + 1 in the other pair
+  const bool la_layout = lookahead_layout(algo, c, n, nb);
+  void* plb[2] = {pl, la_layout ? ar.take((size_t)m * nb * 4) : nullptr};
+  void* pub[2] = {pu, la_layout ? ar.take((size_t)nb * m * 4) : nullptr};
+  bool ahead = la_layout && h->lookahead;
+  if (ahead) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(h->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+      ahead = false;  // one stream under graph capture
+  }
+  if (ahead) RF_CHECK(side_stream(h));
   // BF16 outer updates with fp32 inner buffers: a bf16 mirror of the
   // finished factors (each block rounded once) feeds the outer GEMMs
   const bool bf16m = buffered && c.O == REMIFA_BF16_TC32;
@@ -263,6 +321,11 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
     f8.su = (uint8_t*)ar.take((size_t)n * f8.lds8);
   }
   h->ws_used = ar.off;
+  // split-K regions: [ar.off, +R) for the handle's stream, [ar.off + R,
+  // +R) for the side stream (the arena's end is lowered while they run)
+  const size_t ws_bytes0 = h->ws_bytes;
+  const size_t side_lo = ar.off + gemm_reserve(m, nb), side_hi = side_lo + gemm_reserve(m, nb);
+  if (la_layout) h->ws_bytes = side_lo;
   const int64_t nc = (n - 1) / nb + 1;
   // outer update from the mirrors: C[rows r0.., cols c0..] (M x N) -= L[r0.., 0:K] U[0:K, c0..]
   auto gemm8 = [&](int out_t, int64_t M, int64_t N, int64_t K, int64_t r0, int64_t c0, void* cc,
@@ -287,7 +350,47 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
     return 0;
   };
 
+  // the 16-bit operands of the buffered outer updates: A itself, or its
+  // bf16 mirror
+  void* const obase = bf16m ? b16 : d_a;
+  const int otyp = bf16m ? REMIFA_R_16BF : W;
+  const int64_t old_ = bf16m ? ldb16 : ldda;
+  // side stream, while block k1 - 1 is factored: buffers of block k1 =
+  // A - (blocks 0 .. k1 - 2) (a plain fp32 copy of A for k1 = 1). Waits for
+  // the handle stream's last ev_main (block k1 - 2 finished and mirrored).
+  auto side_prep = [&](int64_t k1) -> int {
+    const int64_t o1 = k1 * nb, in1 = std::min(n - o1, nb), updm1 = m - o1, K1 = o1 - nb;
+    void* blk1 = at(d_a, W, o1, o1, ldda);
+    void *npl = plb[k1 & 1], *npu = pub[k1 & 1];
+    RF_HIP(hipStreamWaitEvent(h->side, h->ev_main, 0));
+    hipStream_t s0 = h->stream;
+    const size_t u0 = h->ws_used, e0 = h->ws_bytes;
+    h->stream = h->side;
+    h->ws_used = side_lo;
+    h->ws_bytes = side_hi;
+    int st = 0;
+    if (K1 == 0) {
+      st = convert_launch(h, W, REMIFA_R_32F, updm1, in1, blk1, ldda, npl, ldpl);
+      if (!st) st = convert_launch(h, W, REMIFA_R_32F, in1, updm1, blk1, ldda, npu, ldpu);
+    } else {
+      const int oin = operand_type(c.O);
+      const void* l1 = at(obase, otyp, o1, 0, old_);
+      const void* u1 = at(obase, otyp, 0, o1, old_);
+      st = gemm_launch_cin(h, c.O, bf16m ? REMIFA_R_16BF : oin, REMIFA_R_32F, updm1, in1, K1,
+                           -1.0f, l1, old_, u1, old_, 1.0f, blk1, W, ldda, npl, ldpl);
+      if (!st)
+        st = gemm_launch_cin(h, c.O, bf16m ? REMIFA_R_16BF : oin, REMIFA_R_32F, in1, updm1, K1,
+                             -1.0f, l1, old_, u1, old_, 1.0f, blk1, W, ldda, npu, ldpu);
+    }
+    if (!st && hipEventRecord(h->ev_side, h->side) != hipSuccess) st = REMIFA_ERR_HIP;
+    h->stream = s0;
+    h->ws_used = u0;
+    h->ws_bytes = e0;
+    return st;
+  };
+
   auto run = [&]() -> int {
+    if (ahead) RF_HIP(hipEventRecord(h->ev_main, h->stream));  // inputs ready
     if (algo == REMIFA_LL) {
       for (int64_t kk = 0; kk < nc; ++kk) {
         const int64_t ofs = kk * nb;
@@ -338,7 +441,26 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
           // (the diagonal block is in both buffers, as in the reference's
           // d_l_tmp / d_u_tmp: L and the diagonal tiles are read from pl,
           // the U part right of each tile from pu)
-          if (c.OP == REMIFA_R_32F && ofs > 0 && !fp8) {
+          void* const cpl = ahead ? plb[kk & 1] : pl;
+          void* const cpu = ahead ? pub[kk & 1] : pu;
+          if (ahead) {
+            if (kk == 0) {
+              RF_CHECK(convert_launch(h, W, REMIFA_R_32F, updm, in, blk, ldda, cpl, ldpl));
+              RF_CHECK(convert_launch(h, W, REMIFA_R_32F, in, updm, blk, ldda, cpu, ldpu));
+            } else {
+              // the side stream left A - (blocks 0 .. kk-2) here; block
+              // kk-1 (nb wide) follows: same products, summed in two parts
+              RF_HIP(hipStreamWaitEvent(h->stream, h->ev_side, 0));
+              const int64_t op = ofs - nb;
+              const void* lr = at(obase, otyp, ofs, op, old_);
+              const void* ur = at(obase, otyp, op, ofs, old_);
+              RF_CHECK(gemm_launch(h, c.O, oin, REMIFA_R_32F, updm, in, nb, -1.0f, lr, old_, ur,
+                                   old_, 1.0f, cpl, ldpl));
+              RF_CHECK(gemm_launch(h, c.O, oin, REMIFA_R_32F, in, updm, nb, -1.0f, lr, old_, ur,
+                                   old_, 1.0f, cpu, ldpu));
+            }
+            if (kk + 1 < nc) RF_CHECK(side_prep(kk + 1));
+          } else if (c.OP == REMIFA_R_32F && ofs > 0 && !fp8) {
             // block column and block row straight into the fp32 buffers:
             // buffer = A (fp16, read by the epilogue) - L U (no separate
             // fp16 -> fp32 copy of the block first)
@@ -357,7 +479,7 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
           // inner factorization in fp32; its fp16 mirror is A itself (the
           // reference passes d_a as the fp16 copy, src/lu_nopiv_ll.cxx:632),
           // so the rounded factors land in A as they are produced
-          RF_CHECK(inner_ll(x, REMIFA_R_32F, updm, in, pl, ldpl, pu, ldpu, blk, ldda, true));
+          RF_CHECK(inner_ll(x, REMIFA_R_32F, updm, in, cpl, ldpl, cpu, ldpu, blk, ldda, true));
           if (fp8) RF_CHECK(mirror8(ofs, in));
           if (bf16m) {
             // round the finished block column (below the diagonal block) and
@@ -370,6 +492,7 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
               RF_CHECK(convert_launch(h, W, REMIFA_R_16BF, in, n - r0, at(d_a, W, ofs, r0, ldda),
                                       ldda, at(b16, REMIFA_R_16BF, ofs, r0, ldb16), ldb16));
           }
+          if (ahead) RF_HIP(hipEventRecord(h->ev_main, h->stream));  // block kk finished
         }
       }
     } else {
@@ -421,6 +544,13 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
   };
   int st = run();
   h->ws_used = 0;
+  h->ws_bytes = ws_bytes0;
+  if (ahead) {
+    // join the side stream (also after a failed launch)
+    if (hipEventRecord(h->ev_side, h->side) != hipSuccess ||
+        hipStreamWaitEvent(h->stream, h->ev_side, 0) != hipSuccess)
+      st = st ? st : REMIFA_ERR_HIP;
+  }
   if (st) return st;
   if (d_info) {
     info_kernel<<<1, 256, 0, h->stream>>>(n, d_a, ldda, W, d_info);

@@ -94,6 +94,12 @@ int remifa_destroy(remifa_handle_t h) {
   else hipDeviceSynchronize();
   if (h->ws && !h->ws_user) hipFree(h->ws);
   for (auto e : h->timer.pool) hipEventDestroy(e);
+  if (h->side) {
+    hipStreamSynchronize(h->side);
+    hipStreamDestroy(h->side);
+  }
+  if (h->ev_main) hipEventDestroy(h->ev_main);
+  if (h->ev_side) hipEventDestroy(h->ev_side);
   delete h;
   return 0;
 }
@@ -150,6 +156,15 @@ int remifa_set_gemm_tile(remifa_handle_t h, int tile) {
   if (!h) return -1;
   if (tile < -1 || tile > 7) return -2;
   h->gemm_tile = tile;
+  return 0;
+}
+
+int remifa_set_lookahead(remifa_handle_t h, int on, int reserve_cus) {
+  if (!h) return -1;
+  if (on != 0 && on != 1) return -2;
+  if (reserve_cus < 0 || reserve_cus > h->num_cu / 2 || reserve_cus % 8) return -3;
+  h->lookahead = on;
+  h->side_reserve = reserve_cus;
   return 0;
 }
 

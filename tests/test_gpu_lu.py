@@ -185,3 +185,42 @@ def test_ll_bf16_outer_updates_match_oracle(handle, gpu, OP, n, nb, ib):
     o = orc.f16_bits_to_f32(orc.lu_nopiv("ll", orc.cfg(W=orc.F16, I=orc.TC32, O=orc.BF16TC32,
                                                         OP=op_or, ib=ib), a16, nb))
     _check(g, o, orc.lu_f64(orc.round_to(orc.F16, a32)), orc.F16)
+
+
+@pytest.mark.parametrize("O", ["tc32", "bf16"])
+def test_ll_lookahead_matches_oracle_and_sequential(handle, gpu, O):
+    """LL with fp32 inner buffers: the lookahead (block k+1's update by the
+    blocks before k on a second stream, block k's part afterwards) computes
+    the same products as one update per block, summed in two parts. Both
+    schedules against the oracle, with a ragged last block, in a
+    caller-owned workspace of exactly remifa_lu_workspace_size bytes."""
+    import torch
+    rf = _rf()
+    n, nb, ib = 700, 128, 32
+    a32 = orc.gen_diagdom_alpha(n, 31, 1.1)
+    a16 = orc.f32_to_f16_bits(a32)
+    oc_o, rf_o = (orc.TC32, rf.TC32) if O == "tc32" else (orc.BF16TC32, rf.BF16_TC32)
+    cfg = rf.api.make_config(W=rf.R_16F, I=rf.TC32, O=rf_o, OP=rf.R_32F, ib=ib)
+    nbytes = rf.lu_workspace_size(handle, rf.LL, cfg, n, n, nb)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=gpu)
+    handle.set_workspace(ws)
+    out = {}
+    try:
+        # (on / off, CUs masked off the second stream)
+        for on, rsv in ((1, 32), (1, 0), (0, 32)):
+            handle.set_lookahead(on, rsv)
+            da = to_dev(a16, gpu, ld=n + 3)
+            assert rf.lu_nopiv(handle, da, nb, algo=rf.LL, cfg=cfg) == 0
+            torch.cuda.synchronize()
+            out[(on, rsv)] = orc.f16_bits_to_f32(to_np(da))
+    finally:
+        handle.set_lookahead(1, 0)
+        handle.set_workspace(None)
+    o = orc.f16_bits_to_f32(orc.lu_nopiv("ll", orc.cfg(W=orc.F16, I=orc.TC32, O=oc_o,
+                                                        OP=orc.F32, ib=ib), a16, nb))
+    f64 = orc.lu_f64(orc.round_to(orc.F16, a32))
+    for f in out.values():
+        _check(f, o, f64, orc.F16)
+    # the masked and unmasked runs differ only in where the workgroups ran
+    assert np.array_equal(out[(1, 32)], out[(1, 0)])
+    assert factor_distance(out[(1, 0)], out[(0, 32)]) <= 8 * U[orc.F16]
