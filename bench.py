@@ -370,7 +370,7 @@ def main():
         tot, cnt = 0, 0
         # (with the lookahead block k's update is two launches of each shape:
         # K = ofs - nb on the second stream, then K = nb for block k-1)
-        la = not a.no_lookahead and a.update != "fp8"
+        la = not a.no_lookahead and (a.update != "fp8" or nb % 128 == 0)
         for k in range(1, (n + nb - 1) // nb):
             ofs = k * nb
             ks = ([ofs - nb] if k >= 2 else []) + [nb] if la else [ofs]
@@ -403,7 +403,7 @@ def main():
                                f"{ {'fp16': 'TC32', 'bf16': 'BF16_TC32', 'fp8': 'FP8_TC32'}[a.update]} "
                                f"MFMA outer updates, + FP64-residual IR to 1e-14, nrhs={a.nrhs}",
                    "n": n, "nb": nb, "ib": a.ib, "algo": a.algo, "alpha": a.alpha,
-                   "nrhs": a.nrhs, "seed": a.seed, "lookahead": algo == rf.LL and not a.no_lookahead and a.update != "fp8",
+                   "nrhs": a.nrhs, "seed": a.seed, "lookahead": algo == rf.LL and not a.no_lookahead and (a.update != "fp8" or nb % 128 == 0),
                    "lookahead_reserved_cus": a.reserve_cus,
                    "precision": (f"W=f16 I=TC32 O={ {'fp16': 'TC32', 'bf16': 'BF16_TC32', 'fp8': 'FP8_TC32'}[a.update]}"
                                  + (" OP=IP=OF=IF=f32" if algo == rf.LL else ""))},

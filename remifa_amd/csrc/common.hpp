@@ -204,6 +204,9 @@ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 // (implemented in the .hip files, called by the drivers and the C-ABI)
 int convert_launch(remifa_handle_t h, int src_t, int dst_t, int64_t m, int64_t n, const void* a,
                    int64_t lda, void* b, int64_t ldb);
+// the handle's second stream (h->side) and its two events, created on
+// first use (drivers.hip)
+int side_stream(remifa_handle_t h);
 int panel_launch(remifa_handle_t h, int t, int ib, int64_t m, int64_t n, const void* d, int64_t ldd,
                  void* l, int64_t ldl, void* u, int64_t ldu, int* stat);
 

@@ -188,9 +188,12 @@ struct Fp8Mirror {
 // LL lookahead layout: fp16 storage, fp32 inner buffers and panels, 16-bit
 // outer operands (TC32 from A itself or BF16_TC32 from its bf16 mirror),
 // at least two outer blocks
+// (FP8_TC32 when nb is a multiple of 128: block k's part of the update
+// starts its K range at a multiple of 128, where the scale words stay
+// 4-byte aligned)
 bool lookahead_layout(int algo, const remifa_lu_config_t& c, int64_t n, int64_t nb) {
   return algo == REMIFA_LL && c.W == REMIFA_R_16F && c.IP == REMIFA_R_32F &&
-         c.OP == REMIFA_R_32F && c.O != REMIFA_FP8_TC32 && n > nb;
+         c.OP == REMIFA_R_32F && (c.O != REMIFA_FP8_TC32 || nb % 128 == 0) && n > nb;
 }
 
 // The lookahead's second stream (default priority), masked to leave
@@ -328,12 +331,13 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
   if (la_layout) h->ws_bytes = side_lo;
   const int64_t nc = (n - 1) / nb + 1;
   // outer update from the mirrors: C[rows r0.., cols c0..] (M x N) -= L[r0.., 0:K] U[0:K, c0..]
+  // (K range [k0, k0 + K), k0 a multiple of 128)
   auto gemm8 = [&](int out_t, int64_t M, int64_t N, int64_t K, int64_t r0, int64_t c0, void* cc,
-                   int64_t ldcc) -> int {
+                   int64_t ldcc, int64_t k0 = 0) -> int {
     if (M <= 0 || N <= 0 || K <= 0) return 0;
-    return fp8_gemm_launch(h, out_t, M, N, K, -1.0f, f8.l8 + r0 * f8.ld8, f8.ld8,
-                           f8.sl + r0 * f8.lds8, f8.lds8, f8.u8 + c0 * f8.ld8, f8.ld8,
-                           f8.su + c0 * f8.lds8, f8.lds8, 1.0f, cc, ldcc);
+    return fp8_gemm_launch(h, out_t, M, N, K, -1.0f, f8.l8 + r0 * f8.ld8 + k0, f8.ld8,
+                           f8.sl + r0 * f8.lds8 + k0 / 32, f8.lds8, f8.u8 + c0 * f8.ld8 + k0,
+                           f8.ld8, f8.su + c0 * f8.lds8 + k0 / 32, f8.lds8, 1.0f, cc, ldcc);
   };
   // quantize the finished block kk: L rows below its diagonal block, U
   // columns right of it (from the stored factors, in W precision)
@@ -369,9 +373,11 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
     h->ws_used = side_lo;
     h->ws_bytes = side_hi;
     int st = 0;
-    if (K1 == 0) {
+    if (K1 == 0 || fp8) {
       st = convert_launch(h, W, REMIFA_R_32F, updm1, in1, blk1, ldda, npl, ldpl);
       if (!st) st = convert_launch(h, W, REMIFA_R_32F, in1, updm1, blk1, ldda, npu, ldpu);
+      if (!st && K1 > 0) st = gemm8(REMIFA_R_32F, updm1, in1, K1, o1, o1, npl, ldpl);
+      if (!st && K1 > 0) st = gemm8(REMIFA_R_32F, in1, updm1, K1, o1, o1, npu, ldpu);
     } else {
       const int oin = operand_type(c.O);
       const void* l1 = at(obase, otyp, o1, 0, old_);
@@ -452,12 +458,17 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
               // kk-1 (nb wide) follows: same products, summed in two parts
               RF_HIP(hipStreamWaitEvent(h->stream, h->ev_side, 0));
               const int64_t op = ofs - nb;
-              const void* lr = at(obase, otyp, ofs, op, old_);
-              const void* ur = at(obase, otyp, op, ofs, old_);
-              RF_CHECK(gemm_launch(h, c.O, oin, REMIFA_R_32F, updm, in, nb, -1.0f, lr, old_, ur,
-                                   old_, 1.0f, cpl, ldpl));
-              RF_CHECK(gemm_launch(h, c.O, oin, REMIFA_R_32F, in, updm, nb, -1.0f, lr, old_, ur,
-                                   old_, 1.0f, cpu, ldpu));
+              if (fp8) {
+                RF_CHECK(gemm8(REMIFA_R_32F, updm, in, nb, ofs, ofs, cpl, ldpl, op));
+                RF_CHECK(gemm8(REMIFA_R_32F, in, updm, nb, ofs, ofs, cpu, ldpu, op));
+              } else {
+                const void* lr = at(obase, otyp, ofs, op, old_);
+                const void* ur = at(obase, otyp, op, ofs, old_);
+                RF_CHECK(gemm_launch(h, c.O, oin, REMIFA_R_32F, updm, in, nb, -1.0f, lr, old_,
+                                     ur, old_, 1.0f, cpl, ldpl));
+                RF_CHECK(gemm_launch(h, c.O, oin, REMIFA_R_32F, in, updm, nb, -1.0f, lr, old_,
+                                     ur, old_, 1.0f, cpu, ldpu));
+              }
             }
             if (kk + 1 < nc) RF_CHECK(side_prep(kk + 1));
           } else if (c.OP == REMIFA_R_32F && ofs > 0 && !fp8) {
@@ -472,7 +483,7 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
             RF_CHECK(convert_launch(h, W, REMIFA_R_32F, updm, in, blk, ldda, pl, ldpl));
             RF_CHECK(convert_launch(h, W, REMIFA_R_32F, in, updm, blk, ldda, pu, ldpu));
           }
-          if (c.OP == REMIFA_R_32F && ofs > 0 && fp8) {
+          if (!ahead && c.OP == REMIFA_R_32F && ofs > 0 && fp8) {
             RF_CHECK(gemm8(REMIFA_R_32F, updm, in, ofs, ofs, ofs, pl, ldpl));
             RF_CHECK(gemm8(REMIFA_R_32F, in, updm, ofs, ofs, ofs, pu, ldpu));
           }

@@ -721,17 +721,35 @@ static int ir_solve_impl(remifa_handle_t h, int trans, int lu_t, int64_t n, int6
   hipStream_t s = h->stream;
   const dim3 gv((unsigned)ceil_div(n, 256), (unsigned)nrhs);
 
-  // ||op(A)||_1 and ||b||_inf per column (fixed for the whole solve)
+  // ||op(A)||_1 and ||b||_inf per column (fixed for the whole solve): on
+  // the handle's second stream (reading A, an HBM-bound pass) while the
+  // diagonal blocks are inverted and the first triangular solves run
+  // (latency-bound); joined before the first residual. One stream under
+  // graph capture.
+  hipStreamCaptureStatus cs_st = hipStreamCaptureStatusNone;
+  const bool two = hipStreamIsCapturing(s, &cs_st) == hipSuccess &&
+                   cs_st == hipStreamCaptureStatusNone && side_stream(h) == 0;
+  hipStream_t sn = s;
+  if (two) {
+    RF_HIP(hipEventRecord(h->ev_main, s));  // inputs ready
+    RF_HIP(hipStreamWaitEvent(h->side, h->ev_main, 0));
+    sn = h->side;
+  }
   if (trans)
-    rowsum_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(n, d_a, lda, cs);
+    rowsum_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, sn>>>(n, d_a, lda, cs);
   else
-    colsum_kernel<<<(unsigned)n, 256, 0, s>>>(n, d_a, lda, cs);
-  maxreduce_kernel<<<1, 256, 0, s>>>(n, cs, nrm + 2 * nrhs);
-  colmax_kernel<<<(unsigned)nrhs, 256, 0, s>>>(n, d_b, ldb, nrm + nrhs);
+    colsum_kernel<<<(unsigned)n, 256, 0, sn>>>(n, d_a, lda, cs);
+  maxreduce_kernel<<<1, 256, 0, sn>>>(n, cs, nrm + 2 * nrhs);
+  colmax_kernel<<<(unsigned)nrhs, 256, 0, sn>>>(n, d_b, ldb, nrm + nrhs);
+  RF_HIP(hipGetLastError());
+  if (two) RF_HIP(hipEventRecord(h->ev_side, sn));
   // x0 = op(LU)^-1 b in fp32
   d2f_kernel<<<gv, 256, 0, s>>>(n, (int)nrhs, d_b, ldb, w, n);
-  RF_CHECK(trsm_invert_launch(h, lu_t, n, d_lu, ldlu, tw));
-  RF_CHECK(getrs_launch(h, lu_t, trans, n, nrhs, d_lu, ldlu, w, n, tw));
+  const int st0 = trsm_invert_launch(h, lu_t, n, d_lu, ldlu, tw);
+  const int st1 = st0 ? st0 : getrs_launch(h, lu_t, trans, n, nrhs, d_lu, ldlu, w, n, tw);
+  // (joined also when a launch failed: the workspace may be reused next)
+  if (two) RF_HIP(hipStreamWaitEvent(s, h->ev_side, 0));
+  RF_CHECK(st1);
   axpy_f2d_kernel<<<gv, 256, 0, s>>>(n, w, n, d_x, ldx, 1);
   RF_HIP(hipGetLastError());
 

@@ -110,8 +110,11 @@ def test_gemm_fp8_from_quantized_operands_f16_output(handle, gpu):
 
 
 @pytest.mark.parametrize("algo,OP,n,nb,ib", [("ll", "f32", 320, 64, 8), ("ll", "f16", 256, 64, 8),
-                                             ("ll", "f32", 500, 96, 32), ("rl", "f16", 256, 64, 8)])
+                                             ("ll", "f32", 500, 96, 32), ("rl", "f16", 256, 64, 8),
+                                             ("ll", "f32", 600, 128, 32)])
 def test_lu_fp8_outer_matches_oracle(handle, gpu, algo, OP, n, nb, ib):
+    """(LL with fp32 buffers and nb = 128: the lookahead schedule, block k's
+    part of each update taken from K offset (k - 1) nb of the e4m3 mirrors)"""
     import torch
     import remifa_amd as rf
     a32 = orc.gen_diagdom_alpha(n, 17, 1.2)
