@@ -101,8 +101,9 @@ int remifa_set_gemm_tile(remifa_handle_t handle, int tile);
  * GEMM's M and N decides. Written by remifa_amd/autotune.py. No reference
  * counterpart. */
 int remifa_set_gemm_rules(remifa_handle_t handle, int count, const int64_t *rules);
-/* Left-looking lookahead (1 = on, the default; 0 = off). With fp32 inner
- * buffers and TC32 / BF16_TC32 outer updates, the update of outer block
+/* Left-looking lookahead (1 = on, the default; 0 = off). With fp16 storage,
+ * fp32 inner buffers and TC32 / BF16_TC32 outer updates (FP8_TC32 when nb
+ * is a multiple of 128), the update of outer block
  * k+1 by the blocks before k runs on a second stream of the handle while
  * block k is factored; block k's own contribution follows on the handle's
  * stream. Both streams are joined before remifa_lu_nopiv returns.

@@ -183,16 +183,11 @@ __global__ void __launch_bounds__(TT) trsm_kernel(int64_t n, int nrhs, const LT*
 
   const int nsteps = tk;  // blocks before this one in dependency order
   const int64_t grc = gr < n ? gr : n - 1;  // in-bounds row for the loads
-  // this lane's right-hand sides (k = cq, cq + 4, ...), loaded now so that
-  // no memory round trip follows the last dependency wait (only this
-  // workgroup writes these rows of b)
-  constexpr int NB4 = (NR + 3) / 4;
-  float bpf[NB4];
-#pragma unroll
-  for (int i = 0; i < NB4; ++i) {
-    const int k = cq + 4 * i;
-    bpf[i] = (k < NR) ? b[grc + (k < nrhs ? k : 0) * ldb] : 0.0f;
-  }
+  // one right-hand side: this lane's entry of b (lanes cq = 0), loaded now
+  // so that no memory round trip follows the last dependency wait (only
+  // this workgroup writes these rows of b). With 16 right-hand sides the
+  // extra registers would spill; those are loaded after the waits.
+  const float bpf = NR == 1 ? b[grc] : 0.0f;
   for (int s0 = 0; s0 < nsteps; s0 += CH) {
     const int cnt = nsteps - s0 < CH ? nsteps - s0 : CH;
     // this lane's entries of the chunk's blocks of T: issued before the wait
@@ -261,13 +256,10 @@ __global__ void __launch_bounds__(TT) trsm_kernel(int64_t n, int nrhs, const LT*
 #pragma unroll
   for (int k = 0; k < NR; ++k) Red[cq][row][k] = acc[k];
   __syncthreads();
-#pragma unroll
-  for (int i = 0; i < NB4; ++i) {
-    const int k = cq + 4 * i;
-    if (k < NR) {
-      const float sum = ((Red[0][row][k] + Red[1][row][k]) + Red[2][row][k]) + Red[3][row][k];
-      Xs[row][k] = (gr < n && k < nrhs) ? bpf[i] - sum : 0.0f;
-    }
+  for (int k = cq; k < NR; k += 4) {
+    const float sum = ((Red[0][row][k] + Red[1][row][k]) + Red[2][row][k]) + Red[3][row][k];
+    const float bv = NR == 1 ? bpf : b[grc + (k < nrhs ? k : 0) * ldb];
+    Xs[row][k] = (gr < n && k < nrhs) ? bv - sum : 0.0f;
   }
   __syncthreads();
   // X_i = (T_ii)^-1 (B_i - ...)
