@@ -361,7 +361,8 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
   const int64_t old_ = bf16m ? ldb16 : ldda;
   // side stream, while block k1 - 1 is factored: buffers of block k1 =
   // A - (blocks 0 .. k1 - 2) (a plain fp32 copy of A for k1 = 1). Waits for
-  // the handle stream's last ev_main (block k1 - 2 finished and mirrored).
+  // the handle stream's last ev_main: block k1 - 1's own update part done,
+  // which follows block k1 - 2's factorization and mirrors.
   auto side_prep = [&](int64_t k1) -> int {
     const int64_t o1 = k1 * nb, in1 = std::min(n - o1, nb), updm1 = m - o1, K1 = o1 - nb;
     void* blk1 = at(d_a, W, o1, o1, ldda);
@@ -470,6 +471,10 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
                                      ur, old_, 1.0f, cpu, ldpu));
               }
             }
+            // the second stream's next update starts after this block's
+            // own part (which then has the CUs to itself); that point also
+            // follows block kk-1's factorization, which the update reads
+            if (kk > 0) RF_HIP(hipEventRecord(h->ev_main, h->stream));
             if (kk + 1 < nc) RF_CHECK(side_prep(kk + 1));
           } else if (c.OP == REMIFA_R_32F && ofs > 0 && !fp8) {
             // block column and block row straight into the fp32 buffers:
