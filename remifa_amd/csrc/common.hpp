@@ -238,11 +238,13 @@ int gemm_pair_launch(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t 
 int fp8_quantize_launch(remifa_handle_t h, int in_t, bool rows, int64_t R, int64_t K,
                         const void* src, int64_t ld, uint8_t* q, int64_t ldq, uint8_t* s,
                         int64_t lds);
-// C = alpha A B + beta C from FP8 operands in that format (k a multiple of 32)
+// C = alpha A B + beta C from FP8 operands in that format (k a multiple of
+// 32); with cin, the beta term reads Cin (fp16 or fp32, cin_t) instead of C
 int fp8_gemm_launch(remifa_handle_t h, int out_t, int64_t m, int64_t n, int64_t k, float alpha,
                     const uint8_t* qa, int64_t ldqa, const uint8_t* sa, int64_t ldsa,
                     const uint8_t* qb, int64_t ldqb, const uint8_t* sb, int64_t ldsb, float beta,
-                    void* c, int64_t ldc);
+                    void* c, int64_t ldc, const void* cin = nullptr, int cin_t = 0,
+                    int64_t ldcin = 0);
 
 int copy2d_launch(remifa_handle_t h, size_t elt, int64_t m, int64_t n, const void* a, int64_t lda,
                   void* b, int64_t ldb);

@@ -331,13 +331,15 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
   if (la_layout) h->ws_bytes = side_lo;
   const int64_t nc = (n - 1) / nb + 1;
   // outer update from the mirrors: C[rows r0.., cols c0..] (M x N) -= L[r0.., 0:K] U[0:K, c0..]
-  // (K range [k0, k0 + K), k0 a multiple of 128)
+  // (K range [k0, k0 + K), k0 a multiple of 128; cin: C = cin - L U, cin
+  // in W precision with leading dimension ldda)
   auto gemm8 = [&](int out_t, int64_t M, int64_t N, int64_t K, int64_t r0, int64_t c0, void* cc,
-                   int64_t ldcc, int64_t k0 = 0) -> int {
+                   int64_t ldcc, int64_t k0 = 0, const void* cin = nullptr) -> int {
     if (M <= 0 || N <= 0 || K <= 0) return 0;
     return fp8_gemm_launch(h, out_t, M, N, K, -1.0f, f8.l8 + r0 * f8.ld8 + k0, f8.ld8,
                            f8.sl + r0 * f8.lds8 + k0 / 32, f8.lds8, f8.u8 + c0 * f8.ld8 + k0,
-                           f8.ld8, f8.su + c0 * f8.lds8 + k0 / 32, f8.lds8, 1.0f, cc, ldcc);
+                           f8.ld8, f8.su + c0 * f8.lds8 + k0 / 32, f8.lds8, 1.0f, cc, ldcc, cin,
+                           W, ldda);
   };
   // quantize the finished block kk: L rows below its diagonal block, U
   // columns right of it (from the stored factors, in W precision)
@@ -374,11 +376,13 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
     h->ws_used = side_lo;
     h->ws_bytes = side_hi;
     int st = 0;
-    if (K1 == 0 || fp8) {
+    if (K1 == 0) {
       st = convert_launch(h, W, REMIFA_R_32F, updm1, in1, blk1, ldda, npl, ldpl);
       if (!st) st = convert_launch(h, W, REMIFA_R_32F, in1, updm1, blk1, ldda, npu, ldpu);
-      if (!st && K1 > 0) st = gemm8(REMIFA_R_32F, updm1, in1, K1, o1, o1, npl, ldpl);
-      if (!st && K1 > 0) st = gemm8(REMIFA_R_32F, in1, updm1, K1, o1, o1, npu, ldpu);
+    } else if (fp8) {
+      // buffer = A (fp16, read by the epilogue) - L U from the e4m3 mirrors
+      st = gemm8(REMIFA_R_32F, updm1, in1, K1, o1, o1, npl, ldpl, 0, blk1);
+      if (!st) st = gemm8(REMIFA_R_32F, in1, updm1, K1, o1, o1, npu, ldpu, 0, blk1);
     } else {
       const int oin = operand_type(c.O);
       const void* l1 = at(obase, otyp, o1, 0, old_);
@@ -484,13 +488,13 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
                                      lbo, 1.0f, blk, W, ldda, pl, ldpl));
             RF_CHECK(gemm_launch_cin(h, c.O, oin, REMIFA_R_32F, in, updm, ofs, -1.0f, lb, lbo, ub,
                                      lbo, 1.0f, blk, W, ldda, pu, ldpu));
+          } else if (c.OP == REMIFA_R_32F && ofs > 0 && fp8) {
+            // likewise from the e4m3 mirrors
+            RF_CHECK(gemm8(REMIFA_R_32F, updm, in, ofs, ofs, ofs, pl, ldpl, 0, blk));
+            RF_CHECK(gemm8(REMIFA_R_32F, in, updm, ofs, ofs, ofs, pu, ldpu, 0, blk));
           } else {
             RF_CHECK(convert_launch(h, W, REMIFA_R_32F, updm, in, blk, ldda, pl, ldpl));
             RF_CHECK(convert_launch(h, W, REMIFA_R_32F, in, updm, blk, ldda, pu, ldpu));
-          }
-          if (!ahead && c.OP == REMIFA_R_32F && ofs > 0 && fp8) {
-            RF_CHECK(gemm8(REMIFA_R_32F, updm, in, ofs, ofs, ofs, pl, ldpl));
-            RF_CHECK(gemm8(REMIFA_R_32F, in, updm, ofs, ofs, ofs, pu, ldpu));
           }
           // inner factorization in fp32; its fp16 mirror is A itself (the
           // reference passes d_a as the fp16 copy, src/lu_nopiv_ll.cxx:632),

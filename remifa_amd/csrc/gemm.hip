@@ -364,11 +364,12 @@ int fp8_quantize_launch(remifa_handle_t h, int in_t, bool rows, int64_t R, int64
 int fp8_gemm_launch(remifa_handle_t h, int out_t, int64_t m, int64_t n, int64_t k, float alpha,
                     const uint8_t* qa, int64_t ldqa, const uint8_t* sa, int64_t ldsa,
                     const uint8_t* qb, int64_t ldqb, const uint8_t* sb, int64_t ldsb, float beta,
-                    void* c, int64_t ldc) {
+                    void* c, int64_t ldc, const void* cin, int cin_t, int64_t ldcin) {
   if (m <= 0 || n <= 0) return 0;
   if (k < 0 || k % 32) return -3;
   if (out_t != REMIFA_R_32F && out_t != REMIFA_R_16F) return -5;
   if (!c || (k > 0 && (!qa || !qb || !sa || !sb))) return -6;
+  if (cin && ((cin_t != REMIFA_R_16F && cin_t != REMIFA_R_32F) || ldcin < m)) return -7;
   const int64_t nkb4 = ceil_div(ceil_div(k, 32), 4) * 4;
   if (ldqa % 16 || ldqb % 16 || (uintptr_t)qa % 16 || (uintptr_t)qb % 16 || ldqa < k ||
       ldqb < k || ldsa % 4 || ldsb % 4 || (uintptr_t)sa % 4 || (uintptr_t)sb % 4 ||
@@ -379,6 +380,7 @@ int fp8_gemm_launch(remifa_handle_t h, int out_t, int64_t m, int64_t n, int64_t 
   g.A = qa, g.lda = ldqa, g.SA = sa, g.ldsa = ldsa;
   g.B = qb, g.ldb = ldqb, g.SB = sb, g.ldsb = ldsb;
   g.alpha = alpha, g.beta = beta, g.C = c, g.ldc = ldc;
+  g.Cin = cin, g.ldcin = ldcin, g.cin_half = cin_t == REMIFA_R_16F;
   g.tiles_m = (int)ceil_div(m, 128);
   g.tiles_n = (int)ceil_div(n, 128);
   constexpr int BK = 128;
@@ -411,10 +413,12 @@ int fp8_gemm_launch(remifa_handle_t h, int out_t, int64_t m, int64_t n, int64_t 
     dim3 rg((unsigned)ceil_div(m, 256), (unsigned)(n < 65535 ? n : 65535));
     if (out_t == REMIFA_R_32F)
       splitk_reduce_kernel<float><<<rg, 256, 0, h->stream>>>(m, n, split, g.W, g.ldw, beta,
-                                                             (float*)c, ldc, nullptr, 0, 0);
+                                                             (float*)c, ldc, g.Cin, g.ldcin,
+                                                             g.cin_half);
     else
       splitk_reduce_kernel<half_t><<<rg, 256, 0, h->stream>>>(m, n, split, g.W, g.ldw, beta,
-                                                              (half_t*)c, ldc, nullptr, 0, 0);
+                                                              (half_t*)c, ldc, g.Cin, g.ldcin,
+                                                              g.cin_half);
     if (hipGetLastError() != hipSuccess) st = REMIFA_ERR_HIP;
     timer_end(h);
   }

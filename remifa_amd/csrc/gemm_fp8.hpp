@@ -32,6 +32,9 @@ struct Fp8Args {
   float alpha, beta;
   void* C;
   int64_t ldc;
+  const void* Cin;  // optional source of the beta term (else C): fp16 or fp32
+  int64_t ldcin;
+  int cin_half;
   float* W;        // split-K partials [split][N][ldw]
   int64_t ldw;
   int64_t kchunk;  // K range per split (multiple of 128)
@@ -192,27 +195,37 @@ __global__ void __launch_bounds__(256, 2) gemm_fp8_kernel(Fp8Args g) {
     __syncthreads();
   }
 
-  // epilogue: lane holds C[m = .. + l32][n = .. + 8*(q/4) + 4*lh + q%4]
-  const OutT* __restrict__ Cr = (const OutT*)g.C;
+  // epilogue: lane holds C[m = .. + l32][n = .. + 8*(q/4) + 4*lh + q%4];
+  // the beta term's source (C, or Cin in fp16 / fp32) is chosen once per
+  // workgroup, outside the loops
+  auto epilogue = [&](const auto* __restrict__ src, int64_t lds_) {
 #pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    const int64_t m = m0 + wm * 64 + i * 32 + l32;
+    for (int i = 0; i < 2; ++i) {
+      const int64_t m = m0 + wm * 64 + i * 32 + l32;
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int64_t nb = n0 + wn * 64 + j * 32 + 4 * lh;
+      for (int j = 0; j < 2; ++j) {
+        const int64_t nb = n0 + wn * 64 + j * 32 + 4 * lh;
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const int64_t n = nb + 8 * (q >> 2) + (q & 3);
-        if (m >= g.M || n >= g.N) continue;
-        if (split) {
-          g.W[((int64_t)blockIdx.z * g.N + n) * g.ldw + m] = g.alpha * acc[i][j][q];
-        } else {
-          const float cv = g.beta != 0.0f ? (float)Cr[m + n * g.ldc] : 0.0f;
-          ((OutT*)g.C)[m + n * g.ldc] = (OutT)__builtin_fmaf(g.alpha, acc[i][j][q], g.beta * cv);
+        for (int q = 0; q < 16; ++q) {
+          const int64_t n = nb + 8 * (q >> 2) + (q & 3);
+          if (m >= g.M || n >= g.N) continue;
+          if (split) {
+            g.W[((int64_t)blockIdx.z * g.N + n) * g.ldw + m] = g.alpha * acc[i][j][q];
+          } else {
+            const float cv = g.beta != 0.0f ? (float)src[m + n * lds_] : 0.0f;
+            ((OutT*)g.C)[m + n * g.ldc] =
+                (OutT)__builtin_fmaf(g.alpha, acc[i][j][q], g.beta * cv);
+          }
         }
       }
     }
-  }
+  };
+  if (split || !g.Cin)
+    epilogue((const OutT*)g.C, g.ldc);
+  else if (g.cin_half)
+    epilogue((const half_t*)g.Cin, g.ldcin);
+  else
+    epilogue((const float*)g.Cin, g.ldcin);
 }
 
 // ---------------------------------------------------------------- quantizer
