@@ -224,3 +224,19 @@ def test_ll_lookahead_matches_oracle_and_sequential(handle, gpu, O):
     # the masked and unmasked runs differ only in where the workgroups ran
     assert np.array_equal(out[(1, 32)], out[(1, 0)])
     assert factor_distance(out[(1, 0)], out[(0, 32)]) <= 8 * U[orc.F16]
+
+
+def test_set_lookahead_rejects_bad_arguments(handle, gpu):
+    """remifa_set_lookahead: on must be 0 / 1; reserve_cus a multiple of 8,
+    at most half the CUs (status -2 / -3, the knob unchanged)"""
+    rf = _rf()
+    from remifa_amd._lib import lib
+    h = handle._h
+    assert lib().remifa_set_lookahead(h, 2, 0) == -2
+    assert lib().remifa_set_lookahead(h, 1, 12) == -3
+    assert lib().remifa_set_lookahead(h, 1, -8) == -3
+    assert lib().remifa_set_lookahead(h, 1, 4096) == -3
+    assert lib().remifa_set_lookahead(None, 1, 0) == -1
+    assert lib().remifa_set_lookahead(h, 1, 0) == 0
+    with pytest.raises(rf.RemifaError):
+        handle.set_lookahead(True, 7)
