@@ -107,13 +107,17 @@ int remifa_set_gemm_rules(remifa_handle_t handle, int count, const int64_t *rule
  * k+1 by the blocks before k runs on a second stream of the handle while
  * block k is factored; block k's own contribution follows on the handle's
  * stream. Both streams are joined before remifa_lu_nopiv returns.
- * reserve_cus (a multiple of 8, at most half the CUs; default 0): CUs
- * masked off the second stream, spread evenly over the XCDs, so that the
+ * reserve_cus (a multiple of 8 in [0, 64]; default 0): CUs masked off the
+ * second stream, reserve_cus / 8 in each of the 8 XCDs, so that the
  * factorization of block k always finds free CUs (on MI355X any mask made
- * the n = 32768 factorization slower, 16 CUs: 1.6x). No reference
- * counterpart (src/lu_nopiv_ll.hxx:208-562 runs the block updates and
- * factorizations in sequence on one stream). */
+ * the n = 32768 factorization slower, 16 CUs: 1.6x). The mask is built for
+ * the 256-CU MI355X layout: a nonzero reserve on a device with another CU
+ * count is rejected (-3). A rejected call leaves both settings unchanged.
+ * No reference counterpart (src/lu_nopiv_ll.hxx:208-562 runs the block
+ * updates and factorizations in sequence on one stream). */
 int remifa_set_lookahead(remifa_handle_t handle, int on, int reserve_cus);
+/* The current lookahead settings (either pointer may be null). */
+int remifa_get_lookahead(remifa_handle_t handle, int *on, int *reserve_cus);
 const char *remifa_last_error(void);
 
 /* Per-kernel timing with hipEvents on the handle's stream. While enabled,
@@ -265,6 +269,29 @@ int remifa_ir_solve_trans(remifa_handle_t h, int lu_t, int64_t n, int64_t nrhs,
                           const double *d_b, int64_t ldb, double *d_x, int64_t ldx, double tol,
                           int maxit, int *iters, double *bwd_err, double *history);
 
+/* ------------------------------------------------------------ error measures */
+/* The reference's error measures (tests/errors.hxx) in FP64 on the GPU, for
+ * A (fp32, n x n) and solutions d_x of A X = B (FP64, n x nrhs), with the
+ * factors d_lu (lu_t REMIFA_R_32F / REMIFA_R_16F, unit L below the
+ * diagonal, U on and above it; may be null). out[0] = the normwise backward
+ * error ||b - A x||_inf / (||A||_1 ||x||_inf + ||b||_inf)
+ * (unsym_backward_error, :186-249), out[1] = the componentwise error
+ * max_i |b - A x|_i / (|A||x| + |L||U||x|)_i (unsym_compwise_error,
+ * :22-182; NaN when d_lu is null), out[2] = the Oettli-Prager form
+ * max_i |b - A x|_i / (|A||x|)_i (printed by the same function), each the
+ * worst over the columns (the reference's compwise error takes one column).
+ * out is host memory (3 doubles); the call synchronises the stream. */
+int remifa_error_measures(remifa_handle_t h, int lu_t, int64_t n, int64_t nrhs, const float *d_a,
+                          int64_t lda, const void *d_lu, int64_t ldlu, const double *d_b,
+                          int64_t ldb, const double *d_x, int64_t ldx, double *out);
+/* Factor error of tests/errors.hxx:251-346 (factor_backward_error) in FP64:
+ * out[0] = max_ij |A - L U|_ij / (|L||U|)_ij (|A - L U|_ij where the
+ * denominator is 0), out[1] = max_ij |A - L U|_ij, out[2] = ||A - L U||_F /
+ * ||A||_F, out[3] = ||A - L U||_F. O(n^3) FP64 work. out is host memory (4
+ * doubles); the call synchronises the stream. */
+int remifa_factor_error(remifa_handle_t h, int lu_t, int64_t n, const float *d_a, int64_t lda,
+                        const void *d_lu, int64_t ldlu, double *out);
+
 /* ------------------------------------------------------------ generators */
 /* The benchmark generator on the GPU: a_ij uniform(-1,1) drawn row by row
  * from libstdc++'s minstd_rand0 seeded with seed (tests/genmat.hxx:68-102
@@ -275,6 +302,44 @@ int remifa_gen_diagdom(remifa_handle_t h, int64_t n, uint64_t seed, double alpha
 /* Seeded uniform(-1,1) m x n block, row-major draw order. */
 int remifa_gen_uniform(remifa_handle_t h, int64_t m, int64_t n, uint64_t seed, float *d_b,
                        int64_t ldb);
+
+/* gen_randmat (tests/common.hxx:298-319): a_ij uniform(-1,1) drawn column
+ * by column (j outer, i inner) from minstd_rand0 seeded with seed (the
+ * reference's default argument is 1). The random matrix behind the
+ * condition-number generator gen_posdef_cond (tests/common.hxx:526-727). */
+int remifa_gen_randmat(remifa_handle_t h, int64_t m, int64_t n, uint64_t seed, float *d_a,
+                       int64_t lda);
+/* gen_arrowhead (tests/common.hxx:391-455) for an n x n matrix: 256 x 256
+ * diagonal blocks and the last 256 rows and columns (the arrow) filled with
+ * uniform(lo, hi) draws of a default-seeded minstd_rand0 in the reference's
+ * loop order (later writes win), zeros elsewhere, a_jj = 512. The harness
+ * uses [lo, hi] = [0, 1] (--arrowhead, --arrowhead-pos) and [-1, 1]
+ * (--arrowhead-pos-neg). */
+int remifa_gen_arrowhead(remifa_handle_t h, int64_t n, float lo, float hi, float *d_a,
+                         int64_t lda);
+
+/* ------------------------------------------------------------ equilibration */
+/* LAPACK sgeequb on the GPU (the reference harness's --scale,
+ * tests/testing_lu_gpu.hxx:288-311 -> src/wrappers.cxx:244-250): row and
+ * column scale factors d_r (m), d_c (n) that are powers of two,
+ * r_i = 1 / 2^trunc(log2 max_j |a_ij|), c_j likewise for the row-scaled
+ * columns; *rowcnd, *colcnd, *amax and *info (0; i for a zero row i; m + j
+ * for a zero column j, 1-based) as LAPACK returns them (host memory). */
+int remifa_geequb(remifa_handle_t h, int64_t m, int64_t n, const float *d_a, int64_t lda,
+                  float *d_r, float *d_c, float *rowcnd, float *colcnd, float *amax, int *info);
+/* LAPACK slaqge on the GPU (src/wrappers.cxx:209-213): scale d_a in place by
+ * rows, columns, both or neither from the geequb results (THRESH = 0.1);
+ * *equed receives 'N', 'R', 'C' or 'B'. Exact for power-of-two factors. */
+int remifa_laqge(remifa_handle_t h, int64_t m, int64_t n, float *d_a, int64_t lda,
+                 const float *d_r, const float *d_c, float rowcnd, float colcnd, float amax,
+                 char *equed);
+/* The harness's conversion check (--check-conv, tests/common.hxx:107-174):
+ * out[0] = max_ij |a_ij - a~_ij| / |a_ij| (|a_ij - a~_ij| where a_ij = 0),
+ * out[1] = max_ij |a_ij - a~_ij|, with a~ the converted copy d_conv
+ * (conv_t REMIFA_R_16F, REMIFA_R_16BF or REMIFA_R_32F) widened to FP64.
+ * out is host memory (2 doubles); the call synchronises the stream. */
+int remifa_conv_error(remifa_handle_t h, int conv_t, int64_t m, int64_t n, const float *d_a,
+                      int64_t lda, const void *d_conv, int64_t ldc, double *out);
 
 /* ------------------------------------------------------------ batched */
 /* Factor `count` independent matrices of orders n[i] (<= 512) stored at

@@ -61,6 +61,10 @@ def _declare(L):
         "orc_gen_diagdom": (None, [i64, i64, vp, i64]),
         "orc_gen_diagdom_alpha": (None, [i64, u64, dp, vp, i64]),
         "orc_gen_uniform": (None, [i64, i64, u64, vp, i64]),
+        "orc_gen_randmat": (None, [i64, i64, u64, vp, i64]),
+        "orc_gen_arrowhead": (None, [i64, i64, C.c_float, C.c_float, vp, i64]),
+        "orc_geequb": (C.c_int, [i64, i64, vp, i64, vp, vp, vp, vp, vp]),
+        "orc_laqge": (C.c_char, [i64, i64, vp, i64, vp, vp, C.c_float, C.c_float, C.c_float]),
         "orc_f32_to_f16": (C.c_uint16, [C.c_float]),
         "orc_f16_to_f32": (C.c_float, [C.c_uint16]),
         "orc_f64_to_f16": (C.c_uint16, [dp]),
@@ -73,6 +77,7 @@ def _declare(L):
         "orc_lu_nopiv_panel": (C.c_int, [C.c_int, i64, i64, i64, vp, i64, vp, i64,
                                          vp, i64, vp]),
         "orc_quantise_e4m3_codes": (None, [i64, i64, vp, i64, vp, vp, C.c_int]),
+        "orc_set_sum_order": (None, [C.c_int]),
         "orc_gemm_sub": (None, [C.c_int, i64, i64, i64, vp, i64, vp, i64, vp, i64,
                                 C.c_int]),
         "orc_lu_nopiv_ll_typed": (C.c_int, [C.POINTER(LuCfg), i64, i64, i64, vp, i64]),
@@ -102,6 +107,11 @@ def _p(a: np.ndarray):
 
 def _fortran(a, dtype):
     return np.asfortranarray(np.asarray(a, dtype=dtype))
+
+
+def set_sum_order(order: int) -> None:
+    """0: k ascending (the restatement); 1: k descending (fp32 sums only)"""
+    lib().orc_set_sum_order(int(order))
 
 
 def set_threads(n: int) -> None:
@@ -142,6 +152,41 @@ def gen_uniform(m: int, n: int, seed: int) -> np.ndarray:
 # ---------------------------------------------------------------- conversions
 _NP = {F32: np.float32, F64: np.float64, F16: np.uint16, BF16: np.uint16,
        E4M3: np.uint8}
+
+
+def gen_randmat(m: int, n: int, seed: int = 1) -> np.ndarray:
+    """tests/common.hxx:298-319 (column-major draws)"""
+    a = np.zeros((m, n), np.float32, order="F")
+    lib().orc_gen_randmat(m, n, seed, _p(a), m)
+    return a
+
+
+def gen_arrowhead(n: int, lo: float = 0.0, hi: float = 1.0) -> np.ndarray:
+    """tests/common.hxx:391-455 (k = 256)"""
+    a = np.zeros((n, n), np.float32, order="F")
+    lib().orc_gen_arrowhead(n, n, lo, hi, _p(a), n)
+    return a
+
+
+def geequb(a):
+    """LAPACK sgeequb restated: (r, c, rowcnd, colcnd, amax, info)"""
+    a = _fortran(a, np.float32)
+    m, n = a.shape
+    r, c = np.zeros(m, np.float32), np.zeros(n, np.float32)
+    out = np.zeros(3, np.float32)
+    info = lib().orc_geequb(m, n, _p(a), m, _p(r), _p(c), _p(out[0:]), _p(out[1:]),
+                            _p(out[2:]))
+    return r, c, float(out[0]), float(out[1]), float(out[2]), int(info)
+
+
+def laqge(a, r, c, rowcnd, colcnd, amax):
+    """LAPACK slaqge restated: (scaled copy of a, equed)"""
+    a = np.array(_fortran(a, np.float32), order="F", copy=True)
+    m, n = a.shape
+    r = np.ascontiguousarray(r, np.float32)
+    c = np.ascontiguousarray(c, np.float32)
+    eq = lib().orc_laqge(m, n, _p(a), m, _p(r), _p(c), rowcnd, colcnd, amax)
+    return a, eq.decode()
 
 
 def convert(a: np.ndarray, src: int, dst: int) -> np.ndarray:

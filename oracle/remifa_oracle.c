@@ -130,6 +130,63 @@ void orc_gen_diagdom_alpha(int64_t n, uint64_t seed, double alpha, float *a,
     }
 }
 
+/* libstdc++ uniform_real_distribution<float>(lo, hi) for one engine state:
+ * generate_canonical<float> * (hi - lo) + lo, all in float. */
+static float minstd_uniform_ab(uint64_t x, float lo, float hi)
+{
+    float sum = (float)(x - 1);
+    float ret = sum / 2147483648.0f;
+    if (ret >= 1.0f) ret = nextafterf(1.0f, 0.0f);
+    return ret * (hi - lo) + lo;
+}
+
+/* tests/common.hxx:298-319 gen_randmat: engine seeded with `seed`,
+ * uniform(-1,1) drawn column by column (j outer, i inner). */
+void orc_gen_randmat(int64_t m, int64_t n, uint64_t seed, float *a, int64_t lda)
+{
+    uint64_t x = orc_minstd_seed(seed);
+    for (int64_t j = 0; j < n; ++j)
+        for (int64_t i = 0; i < m; ++i) {
+            x = (MINSTD_A * x) % MINSTD_M;
+            a[j * lda + i] = minstd_uniform_ab(x, -1.0f, 1.0f);
+        }
+}
+
+/* tests/common.hxx:391-455 gen_arrowhead with k = 256: zero matrix; in each
+ * k x k diagonal block, for j then i over the block, a(i,j) and a(j,i) take
+ * the next two draws of a default-seeded engine (uniform(lo, hi)); then for
+ * every column j and rows i in [max(m - k, j), m): a(i,j) and a(j,i) take
+ * the next two draws (the arrow); finally a_jj = 2k. */
+void orc_gen_arrowhead(int64_t m, int64_t n, float lo, float hi, float *a, int64_t lda)
+{
+    const int64_t k = 256;
+    uint64_t x = 1;
+    for (int64_t j = 0; j < n; ++j)
+        for (int64_t i = 0; i < m; ++i) a[j * lda + i] = 0.0f;
+    int64_t nc = 1 + (n - 1) / k;
+    for (int64_t c = 0; c < nc; ++c) {
+        int64_t je = n < (c + 1) * k ? n : (c + 1) * k;
+        int64_t ie = m < (c + 1) * k ? m : (c + 1) * k;
+        for (int64_t j = c * k; j < je; ++j)
+            for (int64_t i = c * k; i < ie; ++i) {
+                x = (MINSTD_A * x) % MINSTD_M;
+                a[j * lda + i] = minstd_uniform_ab(x, lo, hi);
+                x = (MINSTD_A * x) % MINSTD_M;
+                a[i * lda + j] = minstd_uniform_ab(x, lo, hi);
+            }
+    }
+    for (int64_t j = 0; j < n; ++j) {
+        int64_t i0 = m - k > j ? m - k : j;
+        for (int64_t i = i0; i < m; ++i) {
+            x = (MINSTD_A * x) % MINSTD_M;
+            a[j * lda + i] = minstd_uniform_ab(x, lo, hi);
+            x = (MINSTD_A * x) % MINSTD_M;
+            a[i * lda + j] = minstd_uniform_ab(x, lo, hi);
+        }
+        a[j * lda + j] = (float)(2 * k);
+    }
+}
+
 /* Seeded uniform(-1,1) block (right-hand sides), row-major draw order. */
 void orc_gen_uniform(int64_t m, int64_t n, uint64_t seed, float *b,
                      int64_t ldb)
@@ -550,6 +607,13 @@ static void quantise_e4m3(int64_t m, int64_t k, const float *a, int64_t lda,
     free(sexp);
 }
 
+/* order of the fp32-accumulated product sums of orc_gemm_sub: 0 = k
+ * ascending (the restatement), 1 = k descending. The second order is a
+ * checker's tool: two results with the same rounding points and different
+ * summation orders show how far summation order alone moves the factors. */
+static int g_sum_order = 0;
+void orc_set_sum_order(int order) { g_sum_order = order; }
+
 void orc_gemm_sub(int ctype, int64_t m, int64_t n, int64_t k,
                   const float *a, int64_t lda, const float *b, int64_t ldb,
                   float *c, int64_t ldc, int cprec)
@@ -585,7 +649,8 @@ void orc_gemm_sub(int ctype, int64_t m, int64_t n, int64_t k,
             int64_t jn = j0 + 4 < n ? 4 : n - j0;
             float acc[4][MB];
             memset(acc, 0, sizeof(acc));
-            for (int64_t kk = 0; kk < k; ++kk) {
+            for (int64_t kq = 0; kq < k; ++kq) {
+                const int64_t kk = (g_sum_order && !half_acc) ? k - 1 - kq : kq;
                 const float *ac = A + LDA * kk;
                 for (int64_t jj = 0; jj < jn; ++jj) {
                     float bv = B[kk + LDB * (j0 + jj)];
@@ -1043,4 +1108,81 @@ int orc_ir_solve(int64_t n, int64_t nrhs, const float *a, int64_t lda,
     free(w);
     free(r);
     return done;
+}
+
+/* ------------------------------------------------------------------ */
+/* equilibration (tests/testing_lu_gpu.hxx:288-311 calls LAPACK's      */
+/* sgeequb then slaqge through src/wrappers.cxx:209-250; LAPACK is a    */
+/* third-party dependency, restated here from its published routines)  */
+/* ------------------------------------------------------------------ */
+
+/* RADIX**INT(LOG(x)/LOG(RADIX)) for x > 0, radix 2, with the truncation
+ * toward zero of log2(x) taken exactly from the exponent */
+static float pow2_trunc_log2(float x)
+{
+    int e;
+    float f = frexpf(x, &e); /* x = f 2^e, f in [0.5, 1) */
+    int k = (x >= 1.0f || f == 0.5f) ? e - 1 : e;
+    return ldexpf(1.0f, k);
+}
+
+int orc_geequb(int64_t m, int64_t n, const float *a, int64_t lda, float *r, float *c,
+               float *rowcnd, float *colcnd, float *amax)
+{
+    if (m == 0 || n == 0) {
+        *rowcnd = 1.0f; *colcnd = 1.0f; *amax = 0.0f;
+        return 0;
+    }
+    const float smlnum = FLT_MIN, bignum = 1.0f / FLT_MIN;
+    for (int64_t i = 0; i < m; ++i) r[i] = 0.0f;
+    for (int64_t j = 0; j < n; ++j)
+        for (int64_t i = 0; i < m; ++i) r[i] = fmaxf(r[i], fabsf(a[j * lda + i]));
+    for (int64_t i = 0; i < m; ++i)
+        if (r[i] > 0.0f) r[i] = pow2_trunc_log2(r[i]);
+    float rcmin = bignum, rcmax = 0.0f;
+    for (int64_t i = 0; i < m; ++i) { rcmax = fmaxf(rcmax, r[i]); rcmin = fminf(rcmin, r[i]); }
+    *amax = rcmax;
+    if (rcmin == 0.0f) {
+        for (int64_t i = 0; i < m; ++i)
+            if (r[i] == 0.0f) return (int)(i + 1);
+    }
+    for (int64_t i = 0; i < m; ++i) r[i] = 1.0f / fminf(fmaxf(r[i], smlnum), bignum);
+    *rowcnd = fmaxf(rcmin, smlnum) / fminf(rcmax, bignum);
+    for (int64_t j = 0; j < n; ++j) {
+        float cj = 0.0f;
+        for (int64_t i = 0; i < m; ++i) cj = fmaxf(cj, fabsf(a[j * lda + i]) * r[i]);
+        c[j] = cj > 0.0f ? pow2_trunc_log2(cj) : 0.0f;
+    }
+    rcmin = bignum; rcmax = 0.0f;
+    for (int64_t j = 0; j < n; ++j) { rcmin = fminf(rcmin, c[j]); rcmax = fmaxf(rcmax, c[j]); }
+    if (rcmin == 0.0f) {
+        for (int64_t j = 0; j < n; ++j)
+            if (c[j] == 0.0f) return (int)(m + j + 1);
+    }
+    for (int64_t j = 0; j < n; ++j) c[j] = 1.0f / fminf(fmaxf(c[j], smlnum), bignum);
+    *colcnd = fmaxf(rcmin, smlnum) / fminf(rcmax, bignum);
+    return 0;
+}
+
+char orc_laqge(int64_t m, int64_t n, float *a, int64_t lda, const float *r, const float *c,
+               float rowcnd, float colcnd, float amax)
+{
+    const float thresh = 0.1f;
+    /* SMALL = SLAMCH('S') / SLAMCH('P') = 2^-126 / 2^-23, LARGE = 1 / SMALL */
+    const float small = ldexpf(1.0f, -103), large = 1.0f / small;
+    if (m <= 0 || n <= 0) return 'N';
+    if (rowcnd >= thresh && amax >= small && amax <= large) {
+        if (colcnd >= thresh) return 'N';
+        for (int64_t j = 0; j < n; ++j)
+            for (int64_t i = 0; i < m; ++i) a[j * lda + i] = c[j] * a[j * lda + i];
+        return 'C';
+    }
+    if (colcnd >= thresh) {
+        for (int64_t j = 0; j < n; ++j)
+            for (int64_t i = 0; i < m; ++i) a[j * lda + i] = r[i] * a[j * lda + i];
+        return 'R';
+    }
+    for (int64_t j = 0; j < n; ++j)
+        for (int64_t i = 0; i < m; ++i) a[j * lda + i] = (c[j] * r[i]) * a[j * lda + i];
+    return 'B';
 }

@@ -40,6 +40,11 @@ void orc_gen_diagdom_alpha(int64_t n, uint64_t seed, double alpha,
                            float *a, int64_t lda);
 void orc_gen_uniform(int64_t m, int64_t n, uint64_t seed, float *b,
                      int64_t ldb);
+/* tests/common.hxx:298-319 gen_randmat (column-major draws, uniform(-1,1)) */
+void orc_gen_randmat(int64_t m, int64_t n, uint64_t seed, float *a, int64_t lda);
+/* tests/common.hxx:391-455 gen_arrowhead (k = 256, entries uniform(lo,hi),
+ * a_jj = 2k) */
+void orc_gen_arrowhead(int64_t m, int64_t n, float lo, float hi, float *a, int64_t lda);
 
 /* ---- scalar conversions (IEEE round to nearest even) ---- */
 uint16_t orc_f32_to_f16(float x);
@@ -68,6 +73,8 @@ int orc_lu_nopiv_panel(int prec, int64_t ib, int64_t m, int64_t n,
  * and codes[r * k + kk] = e4m3(a / 2^sexp), nearest even, saturating. */
 void orc_quantise_e4m3_codes(int64_t m, int64_t k, const float *a, int64_t lda, uint8_t *codes,
                              int32_t *sexp, int rows);
+/* 0: k ascending (default), 1: k descending for fp32-accumulated sums */
+void orc_set_sum_order(int order);
 void orc_gemm_sub(int ctype, int64_t m, int64_t n, int64_t k,
                   const float *a, int64_t lda, const float *b, int64_t ldb,
                   float *c, int64_t ldc, int cprec);
@@ -131,6 +138,16 @@ int orc_ir_solve(int64_t n, int64_t nrhs, const float *a, int64_t lda,
                  const float *lu, int64_t ldlu, const double *b, int64_t ldb,
                  double *x, int64_t ldx, double tol, int maxit,
                  double *hist);
+
+/* ---- equilibration: LAPACK sgeequb / slaqge (third party), called by the
+ * reference harness at tests/testing_lu_gpu.hxx:288-311 ---- */
+/* power-of-two row / column scalings; returns LAPACK's INFO (0, i for a
+ * zero row i, m + j for a zero column j, 1-based) */
+int orc_geequb(int64_t m, int64_t n, const float *a, int64_t lda, float *r, float *c,
+               float *rowcnd, float *colcnd, float *amax);
+/* scale a in place as slaqge does; returns EQUED ('N', 'R', 'C', 'B') */
+char orc_laqge(int64_t m, int64_t n, float *a, int64_t lda, const float *r, const float *c,
+               float rowcnd, float colcnd, float amax);
 
 /* threads used by the OpenMP loops (0 = OpenMP default) */
 void orc_set_threads(int nthreads);

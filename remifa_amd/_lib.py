@@ -50,6 +50,7 @@ SIGNATURES = {
     "remifa_set_tuning": [_vp, _i, _i],
     "remifa_set_gemm_tile": [_vp, _i],
     "remifa_set_lookahead": [_vp, _i, _i],
+    "remifa_get_lookahead": [_vp, _pi, _pi],
     "remifa_last_error": [],
     "remifa_timing_enable": [_vp, _i],
     "remifa_timing_query": [_vp, _i, _pd, _pi64, _pd],
@@ -81,8 +82,17 @@ SIGNATURES = {
     "remifa_ir_solve_trans": [_vp, _i, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _d,
                               _i, _pi, _pd, _vp],
     "remifa_set_gemm_rules": [_vp, _i, _vp],
+    "remifa_error_measures": [_vp, _i, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64,
+                              _pd],
+    "remifa_factor_error": [_vp, _i, _i64, _vp, _i64, _vp, _i64, _pd],
     "remifa_gen_diagdom": [_vp, _i64, _u64, _d, _vp, _i64],
     "remifa_gen_uniform": [_vp, _i64, _i64, _u64, _vp, _i64],
+    "remifa_gen_randmat": [_vp, _i64, _i64, _u64, _vp, _i64],
+    "remifa_gen_arrowhead": [_vp, _i64, _f, _f, _vp, _i64],
+    "remifa_geequb": [_vp, _i64, _i64, _vp, _i64, _vp, _vp, C.POINTER(_f), C.POINTER(_f),
+                      C.POINTER(_f), _pi],
+    "remifa_laqge": [_vp, _i64, _i64, _vp, _i64, _vp, _vp, _f, _f, _f, C.c_char_p],
+    "remifa_conv_error": [_vp, _i, _i64, _i64, _vp, _i64, _vp, _i64, _pd],
     "remifa_lu_nopiv_batched": [_vp, _i, _i64, _vp, _vp, _vp, _vp, _i],
     "remifa_getrs_nopiv_batched": [_vp, _i64, _vp, _vp, _vp, _vp, _i],
 }

@@ -317,6 +317,33 @@ def ir_solve(handle, a: torch.Tensor, lu: torch.Tensor, b: torch.Tensor, x: torc
     return it.value, be.value, list(hist)[: k + 1]
 
 
+# ------------------------------------------------------------------ error measures
+def error_measures(handle, a: torch.Tensor, x: torch.Tensor, b: torch.Tensor,
+                   lu: torch.Tensor | None = None) -> dict:
+    """tests/errors.hxx measures of the solutions x (FP64) of A x = b on the
+    GPU: normwise backward error, componentwise error with |L||U| (needs the
+    factors lu), Oettli-Prager form."""
+    n = a.shape[0]
+    nrhs = b.shape[1] if b.dim() == 2 else 1
+    out = (C.c_double * 3)()
+    lu_t = dtype_code(lu) if lu is not None else R_32F
+    check(lib().remifa_error_measures(_h(handle), lu_t, n, nrhs, _p(a), _ld(a), _p(lu),
+                                      _ld(lu) if lu is not None else n, _p(b), _ld(b), _p(x),
+                                      _ld(x), out), "remifa_error_measures")
+    return {"nwerr": out[0], "cwerr": out[1], "cwerr_ax": out[2]}
+
+
+def factor_error(handle, a: torch.Tensor, lu: torch.Tensor) -> dict:
+    """tests/errors.hxx:251-346 factor error of the in-place factors lu of
+    a, on the GPU in FP64."""
+    n = a.shape[0]
+    out = (C.c_double * 4)()
+    check(lib().remifa_factor_error(_h(handle), dtype_code(lu), n, _p(a), _ld(a), _p(lu),
+                                    _ld(lu), out), "remifa_factor_error")
+    return {"factor_cwerr": out[0], "factor_abs": out[1], "factor_rel_fro": out[2],
+            "factor_fro": out[3]}
+
+
 # ------------------------------------------------------------------ generators
 def gen_diagdom(handle, a: torch.Tensor, seed: int = 1, alpha: float = 0.0):
     """Fill the n x n fp32 column-major `a`: uniform(-1,1) off-diagonal from the
@@ -324,6 +351,51 @@ def gen_diagdom(handle, a: torch.Tensor, seed: int = 1, alpha: float = 0.0):
     (alpha <= 0: gen_diagdom of tests/genmat.hxx, a_ii = n, default seed)."""
     n = a.shape[0]
     check(lib().remifa_gen_diagdom(_h(handle), n, seed, alpha, _p(a), _ld(a)), "remifa_gen_diagdom")
+
+
+def gen_randmat(handle, a: torch.Tensor, seed: int = 1):
+    """gen_randmat (tests/common.hxx:298-319): uniform(-1,1), column-major
+    draws, into the fp32 column-major `a`."""
+    m, n = a.shape
+    check(lib().remifa_gen_randmat(_h(handle), m, n, seed, _p(a), _ld(a)), "remifa_gen_randmat")
+
+
+def gen_arrowhead(handle, a: torch.Tensor, lo: float = 0.0, hi: float = 1.0):
+    """gen_arrowhead (tests/common.hxx:391-455, k = 256) into the square
+    fp32 column-major `a`."""
+    n = a.shape[0]
+    check(lib().remifa_gen_arrowhead(_h(handle), n, lo, hi, _p(a), _ld(a)),
+          "remifa_gen_arrowhead")
+
+
+def geequb(handle, a: torch.Tensor, r: torch.Tensor, c: torch.Tensor) -> dict:
+    """LAPACK sgeequb on the GPU: power-of-two scalings into r (m) and c (n);
+    returns rowcnd, colcnd, amax, info."""
+    m, n = a.shape
+    rc, cc, am, info = C.c_float(), C.c_float(), C.c_float(), C.c_int()
+    check(lib().remifa_geequb(_h(handle), m, n, _p(a), _ld(a), _p(r), _p(c), C.byref(rc),
+                              C.byref(cc), C.byref(am), C.byref(info)), "remifa_geequb")
+    return {"rowcnd": rc.value, "colcnd": cc.value, "amax": am.value, "info": info.value}
+
+
+def laqge(handle, a: torch.Tensor, r: torch.Tensor, c: torch.Tensor, rowcnd: float,
+          colcnd: float, amax: float) -> str:
+    """LAPACK slaqge on the GPU: scales `a` in place; returns EQUED."""
+    m, n = a.shape
+    eq = C.create_string_buffer(1)
+    check(lib().remifa_laqge(_h(handle), m, n, _p(a), _ld(a), _p(r), _p(c), rowcnd, colcnd,
+                             amax, eq), "remifa_laqge")
+    return eq.raw.decode()
+
+
+def conv_error(handle, a: torch.Tensor, conv: torch.Tensor) -> dict:
+    """--check-conv (tests/common.hxx:107-174): relative and absolute
+    conversion error of the copy `conv` (fp16 / bf16 / fp32) of `a`."""
+    m, n = a.shape
+    out = (C.c_double * 2)()
+    check(lib().remifa_conv_error(_h(handle), dtype_code(conv), m, n, _p(a), _ld(a), _p(conv),
+                                  _ld(conv), out), "remifa_conv_error")
+    return {"rel": out[0], "abs": out[1]}
 
 
 def gen_uniform(handle, b: torch.Tensor, seed: int):

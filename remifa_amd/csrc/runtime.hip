@@ -162,9 +162,19 @@ int remifa_set_gemm_tile(remifa_handle_t h, int tile) {
 int remifa_set_lookahead(remifa_handle_t h, int on, int reserve_cus) {
   if (!h) return -1;
   if (on != 0 && on != 1) return -2;
-  if (reserve_cus < 0 || reserve_cus > h->num_cu / 2 || reserve_cus % 8) return -3;
+  // the mask (drivers.hip, side_stream) takes reserve / 8 CUs of each of the
+  // 8 XCDs of the 256-CU layout: at most 8 per XCD
+  if (reserve_cus < 0 || reserve_cus > 64 || reserve_cus % 8) return -3;
+  if (reserve_cus > 0 && h->num_cu != 256) return -3;
   h->lookahead = on;
   h->side_reserve = reserve_cus;
+  return 0;
+}
+
+int remifa_get_lookahead(remifa_handle_t h, int* on, int* reserve_cus) {
+  if (!h) return -1;
+  if (on) *on = h->lookahead;
+  if (reserve_cus) *reserve_cus = h->side_reserve;
   return 0;
 }
 

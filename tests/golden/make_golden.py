@@ -8,7 +8,13 @@
                    boundary values, from numpy's float16 and torch's bfloat16 /
                    float8_e4m3fn casts (independent of the oracle);
   ir_trace.json -- backward error per refinement step of the oracle for one
-                   seeded n=256 case (a regression fixture of the oracle).
+                   seeded n=256 case (a regression fixture of the oracle);
+  generators.json -- libstdc++ uniform_real_distribution<float>(0, 1) draws
+                   and FNV-1a hashes of the reference's gen_arrowhead and
+                   gen_randmat matrices (tests/common.hxx:298-319, 391-455)
+                   built by minstd_ref.cpp with the real std:: engine;
+  geequb.json   -- LAPACK sgeequb (scipy's bundled LAPACK) on seeded
+                   badly scaled matrices: r, c, rowcnd, colcnd, amax, info.
 
 usage: python tests/golden/make_golden.py
 """
@@ -37,6 +43,40 @@ def minstd_fixture():
                                text=True)
             out[str(seed)] = r.stdout.split()
     return out
+
+
+ARROW_CASES = [(300, 0.0, 1.0), (300, -1.0, 1.0), (600, 0.0, 1.0), (200, -1.0, 1.0)]
+RANDMAT_CASES = [(37, 29, 5), (256, 256, 1), (1000, 300, 1)]
+
+
+def generators_fixture():
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "minstd_ref")
+        subprocess.run(["g++", "-O2", "-std=c++17", os.path.join(HERE, "minstd_ref.cpp"),
+                        "-o", exe], check=True)
+        run = lambda *a: subprocess.run([exe, *map(str, a)], check=True, capture_output=True,
+                                        text=True).stdout.split()
+        return {
+            "uniform_0_1": run("uniform", 0.0, 1.0, 64),
+            "arrowhead": [[n, lo, hi, run("arrowhead", n, lo, hi)[0]] for n, lo, hi in ARROW_CASES],
+            "randmat": [[m, n, s, run("randmat", m, n, s)[0]] for m, n, s in RANDMAT_CASES],
+        }
+
+
+def geequb_fixture():
+    import scipy.linalg.lapack as lapack
+    rng = np.random.default_rng(9)
+    cases = []
+    for m, n in ((40, 40), (17, 33), (64, 50)):
+        a = rng.uniform(-1, 1, (m, n)) * 10.0 ** rng.integers(-6, 7, (m, 1))
+        a *= 10.0 ** rng.integers(-3, 4, (1, n))
+        a = np.asfortranarray(a.astype(np.float32))
+        r, c, rowcnd, colcnd, amax, info = lapack.sgeequb(a)
+        cases.append({"a_bits": [int(v) for v in a.ravel(order="F").view(np.uint32)],
+                      "m": m, "n": n, "r_bits": [int(v) for v in r.view(np.uint32)],
+                      "c_bits": [int(v) for v in c.view(np.uint32)], "rowcnd": float(rowcnd),
+                      "colcnd": float(colcnd), "amax": float(amax), "info": int(info)})
+    return {"source": "scipy.linalg.lapack.sgeequb", "cases": cases}
 
 
 def boundary_values():
@@ -91,6 +131,10 @@ def main():
         json.dump(convert_fixture(), f)
     with open(os.path.join(HERE, "ir_trace.json"), "w") as f:
         json.dump(ir_fixture(), f, indent=1)
+    with open(os.path.join(HERE, "generators.json"), "w") as f:
+        json.dump(generators_fixture(), f, indent=1)
+    with open(os.path.join(HERE, "geequb.json"), "w") as f:
+        json.dump(geequb_fixture(), f)
     print("fixtures written to", HERE)
 
 

@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -38,7 +40,6 @@ def test_python_package_fails_loudly_without_library(tmp_path, monkeypatch):
     import remifa_amd._lib as L
     monkeypatch.setattr(L, "_lib", None)
     monkeypatch.setattr(L, "library_path", str(tmp_path / "missing.so"))
-    import pytest
     with pytest.raises(ImportError):
         L.lib()
 

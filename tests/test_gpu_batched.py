@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import oracle as orc
+from helpers import assert_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -64,12 +65,13 @@ def test_batched_lu_matches_oracle(handle, gpu):
     c = orc.cfg(W=orc.F32, I=orc.TC32, O=orc.TC32, ib=32)
     for a, lu in zip(mats, got):
         o = orc.lu_nopiv("rl", c, a, 32)
+        # same rounding points as the oracle; only the order of the fp32
+        # sums inside the matrix-core update differs: entrywise parity
+        assert_parity(lu, o, f"batched n={a.shape[0]}")
         f64 = orc.lu_f64(a)
         e_o = np.max(np.abs(o - f64)) / np.max(np.abs(f64))
-        d = np.max(np.abs(lu - o)) / np.max(np.abs(o))
-        # same rounding points as the oracle; only the order of the fp32
-        # sums inside the matrix-core update differs
-        assert d <= 8 * max(e_o, 2.0**-24), (a.shape, d, e_o)
+        e_g = np.max(np.abs(lu - f64)) / np.max(np.abs(f64))
+        assert e_g <= 8 * max(e_o, 2.0**-24), (a.shape, e_g, e_o)
 
 
 def test_batched_bf16_update(handle, gpu):
@@ -128,3 +130,65 @@ def test_batched_zero_pivot_reported(handle, gpu):
     mats = [orc.gen_diagdom_alpha(50, 5, 1.2), a, one]
     _, info = _run(handle, gpu, mats)
     assert info[0] == 0 and info[1] == 41 and info[2] == 1
+
+
+def test_batched_lu_and_solve_in_hip_graph(handle, gpu):
+    """remifa_lu_nopiv_batched + remifa_getrs_nopiv_batched captured in one
+    HIP graph (the queue counter is reset by a kernel inside the graph, no
+    memset node), replayed four times with eager work and synchronisations
+    between the replays: every replay's factors and solutions equal the
+    eager call's bit for bit, and every matrix matches the oracle."""
+    import torch
+    import remifa_amd as rf
+    rng = np.random.default_rng(21)
+    sizes = list(rng.integers(16, 513, 40)) + [16, 512]
+    sizes = sorted(sizes, reverse=True)  # largest first, as the bench orders them
+    mats = _batch(sizes, 700)
+    buf, ptrs, n_t, ld_t, offs, lds = _pack(mats, gpu)
+    a0 = buf.clone()
+    bs = [rng.uniform(-1, 1, m.shape[0]).astype(np.float32) for m in mats]
+    boffs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    b0 = torch.from_numpy(np.concatenate(bs)).to(gpu)
+    bw = b0.clone()
+    bptrs = torch.tensor([bw.data_ptr() + 4 * int(o) for o in boffs], dtype=torch.int64,
+                         device=gpu)
+    info = torch.full((len(mats),), -7, dtype=torch.int32, device=gpu)
+
+    def run(h):
+        rf.lu_nopiv_batched(h, rf.TC32, n_t, ld_t, ptrs, info, 512)
+        rf.getrs_nopiv_batched(h, n_t, ld_t, ptrs, bptrs, 512)
+
+    run(handle)  # eager (also sizes the workspace before the capture)
+    torch.cuda.synchronize()
+    eager_f, eager_x, eager_i = buf.clone(), bw.clone(), info.clone()
+    gs = torch.cuda.Stream()
+    graph = torch.cuda.CUDAGraph()
+    old = handle.stream
+    try:
+        handle.set_stream(gs)
+        with torch.cuda.graph(graph, stream=gs):
+            run(handle)
+    finally:
+        handle.set_stream(old)
+    for rep in range(4):
+        buf.copy_(a0)
+        bw.copy_(b0)
+        info.fill_(-7)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(gs):
+            graph.replay()
+        torch.cuda.synchronize()
+        # eager work between the replays
+        _ = (buf[:1000] * 2.0).sum().item()
+        assert torch.equal(info, eager_i), rep
+        assert torch.equal(buf.view(torch.int32), eager_f.view(torch.int32)), rep
+        assert torch.equal(bw.view(torch.int32), eager_x.view(torch.int32)), rep
+    got = _unpack(buf, mats, offs, lds)
+    c = orc.cfg(W=orc.F32, I=orc.TC32, O=orc.TC32, ib=32)
+    xs = bw.cpu().numpy()
+    for i, (a, lu) in enumerate(zip(mats, got)):
+        assert_parity(lu, orc.lu_nopiv("rl", c, a, 32), f"graph batched n={a.shape[0]}")
+        n = a.shape[0]
+        x = xs[boffs[i]:boffs[i] + n]
+        want = orc.getrs_f32(lu, bs[i])[:, 0]
+        assert np.max(np.abs(x - want)) <= 1e-5 * max(np.max(np.abs(want)), 1.0)

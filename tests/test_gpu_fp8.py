@@ -109,14 +109,19 @@ def test_gemm_fp8_from_quantized_operands_f16_output(handle, gpu):
     assert np.all(np.abs(got - want) <= tol)
 
 
+@pytest.mark.parametrize("split", [0, 4])
 @pytest.mark.parametrize("algo,OP,n,nb,ib", [("ll", "f32", 320, 64, 8), ("ll", "f16", 256, 64, 8),
                                              ("ll", "f32", 500, 96, 32), ("rl", "f16", 256, 64, 8),
                                              ("ll", "f32", 600, 128, 32)])
-def test_lu_fp8_outer_matches_oracle(handle, gpu, algo, OP, n, nb, ib):
+def test_lu_fp8_outer_matches_oracle(handle, gpu, algo, OP, n, nb, ib, split):
     """(LL with fp32 buffers and nb = 128: the lookahead schedule, block k's
-    part of each update taken from K offset (k - 1) nb of the e4m3 mirrors)"""
+    part of each update taken from K offset (k - 1) nb of the e4m3 mirrors;
+    split = 4 forces split-K, so that the C-input updates go through the
+    split-K reduction). Entrywise parity with the oracle (tests/helpers.py),
+    then the error level against the FP64 LU."""
     import torch
     import remifa_amd as rf
+    from helpers import assert_parity
     a32 = orc.gen_diagdom_alpha(n, 17, 1.2)
     a16 = orc.f32_to_f16_bits(a32)
     op_rf = rf.R_32F if OP == "f32" else rf.R_16F
@@ -128,16 +133,21 @@ def test_lu_fp8_outer_matches_oracle(handle, gpu, algo, OP, n, nb, ib):
         cfg = rf.api.make_config(W=rf.R_16F, I=rf.TC32, O=rf.FP8_TC32, OP=op_rf, ib=ib)
         oc = orc.cfg(W=orc.F16, I=orc.TC32, O=orc.FP8TC32, OP=op_or, ib=ib)
     da = to_dev(a16, gpu, ld=n + 3)
-    info = rf.lu_nopiv(handle, da, nb, algo=rf.LL if algo == "ll" else rf.RL, cfg=cfg)
-    torch.cuda.synchronize()
+    try:
+        handle.set_tuning(0, split)
+        info = rf.lu_nopiv(handle, da, nb, algo=rf.LL if algo == "ll" else rf.RL, cfg=cfg)
+        torch.cuda.synchronize()
+    finally:
+        handle.set_tuning(0, 0)
     assert info == 0
-    g = orc.f16_bits_to_f32(to_np(da))
-    o = orc.f16_bits_to_f32(orc.lu_nopiv(algo, oc, a16, nb))
+    graw = to_np(da)
+    oraw = orc.lu_nopiv(algo, oc, a16, nb)
+    assert_parity(graw, oraw, f"fp8 {algo} OP={OP} n={n} nb={nb} split={split}")
+    g, o = orc.f16_bits_to_f32(graw), orc.f16_bits_to_f32(oraw)
     f64 = orc.lu_f64(orc.round_to(orc.F16, a32))
     e_o = factor_distance(o, f64)
-    d = factor_distance(g, o)
-    tol = 8 * max(e_o, 2.0**-11)
-    assert d <= tol, f"GPU vs oracle {d:.3e} > {tol:.3e} (oracle vs FP64 {e_o:.3e})"
+    e_g = factor_distance(g, f64)
+    assert e_g <= 8 * max(e_o, 2.0**-11), f"GPU vs FP64 {e_g:.3e} (oracle vs FP64 {e_o:.3e})"
 
 
 def test_fp8_update_ir_reaches_1e14(handle, gpu):

@@ -2,17 +2,22 @@
 
 For a precision tuple the GPU and the oracle perform the same operations
 with the same rounding points; they differ only in the summation order inside
-the GEMM updates (matrix cores vs. k-ordered fma). Tolerance (written in the
-test): the distance between GPU and oracle factors must not exceed
-8 * max(distance(oracle, FP64 LU), unit roundoff of the storage precision),
-i.e. the GPU result is as close to the oracle as the oracle is to the exact
-factors. Each factor is normalised by its own magnitude.
+the GEMM updates (matrix cores vs. k-ordered fma). Two assertions per case
+(tolerances written in tests/helpers.py):
+ 1. entrywise, in the storage format: at least F16_MIN_EQUAL (fp16) /
+    F32_MIN_EQUAL (fp32) of the entries bit-identical to the oracle's and
+    none more than F16_MAX_ULP / F32_MAX_ULP units in the last place away
+    (a misplaced rounding point moves entries by tens to thousands of ulps:
+    tests/test_oracle.py::test_parity_rule_rejects_planted_rounding_mistakes);
+ 2. error level: the GPU factors are as close to the FP64 LU as the
+    oracle's: distance(GPU, FP64) <= 8 max(distance(oracle, FP64), u_W),
+    each factor normalised by its own magnitude.
 """
 import numpy as np
 import pytest
 
 import oracle as orc
-from helpers import factor_distance, to_dev, to_np
+from helpers import assert_parity, factor_distance, to_dev, to_np
 
 pytestmark = pytest.mark.gpu
 
@@ -28,11 +33,14 @@ def _oc(rf_code):
     return {0: orc.F32, 1: orc.F16}[rf_code]
 
 
-def _ocomp(c):
-    return c  # the compute-type codes are shared by the C ABI and the oracle
+def _f(raw):
+    """storage-format factors (fp16 bits or fp32) as float32 values"""
+    return orc.f16_bits_to_f32(raw) if raw.dtype == np.uint16 else raw
 
 
 def _run_gpu_helper(handle, gpu, name, a32, nb):
+    """one of the 12 reference helpers on the GPU; returns the factors in
+    storage format (fp16 bits or fp32) and info"""
     import torch
     rf = _rf()
     row = [r for r in rf.REFERENCE_HELPERS if r[0] == name][0]
@@ -51,25 +59,23 @@ def _run_gpu_helper(handle, gpu, name, a32, nb):
     else:
         fn(handle, n, n, nb, da, ld, info)
     torch.cuda.synchronize()
-    out = to_np(da)
-    return (orc.f16_bits_to_f32(out) if W == rf.R_16F else out), int(info.item())
+    return to_np(da), int(info.item())
 
 
 def _oracle(row, a32, nb, algo="ll"):
     _, _, W, I, O, OP, IP, OF, IF = row
     c = orc.LuCfg(_oc(W), I, O, _oc(OP), _oc(IP), _oc(OF), _oc(IF), 8)
     a = a32 if c.W == orc.F32 else orc.f32_to_f16_bits(a32)
-    f = orc.lu_nopiv(algo, c, a, nb)
-    return (orc.f16_bits_to_f32(f) if c.W == orc.F16 else f), c
+    return orc.lu_nopiv(algo, c, a, nb), c
 
 
-def _check(gpu_f, ora_f, f64, W):
-    e_o = factor_distance(ora_f, f64)
-    d = factor_distance(gpu_f, ora_f)
-    tol = 8 * max(e_o, U[W])
-    assert d <= tol, f"GPU vs oracle {d:.3e} > tol {tol:.3e} (oracle vs FP64 {e_o:.3e})"
-    e_g = factor_distance(gpu_f, f64)
-    assert e_g <= 8 * max(e_o, U[W])
+def _check(gpu_raw, ora_raw, f64, W, label=""):
+    """entrywise parity in storage format, then the error level"""
+    assert_parity(gpu_raw, ora_raw, label)
+    g, o = _f(gpu_raw), _f(ora_raw)
+    e_o = factor_distance(o, f64)
+    e_g = factor_distance(g, f64)
+    assert e_g <= 8 * max(e_o, U[W]), f"{label}: GPU vs FP64 {e_g:.3e}, oracle vs FP64 {e_o:.3e}"
 
 
 HELPERS = ["lu_nopiv_ll_f32", "lu_nopiv_ll_f32_fp32xtc32", "lu_nopiv_ll_f32_tc32xtc32",
@@ -93,7 +99,7 @@ def test_reference_helper_matches_oracle(handle, gpu, name, n, nb, alpha):
     g, info = _run_gpu_helper(handle, gpu, name, a32, nb)
     o, c = _oracle(row, a32, nb)
     assert info == 0
-    _check(g, o, f64, c.W)
+    _check(g, o, f64, c.W, f"{name} n={n} nb={nb} alpha={alpha}")
 
 
 @pytest.mark.parametrize("W,I,O,nb", [(0, 0, 0, 64), (0, 0, 2, 64), (0, 2, 2, 32), (1, 2, 2, 64),
@@ -112,12 +118,10 @@ def test_rl_matches_oracle(handle, gpu, W, I, O, nb):
     info = rf.lu_nopiv(handle, da, nb, algo=rf.RL, cfg=cfg)
     torch.cuda.synchronize()
     g = to_np(da)
-    g = orc.f16_bits_to_f32(g) if ow == orc.F16 else g
     o = orc.lu_nopiv("rl", c, a_host, nb)
-    o = orc.f16_bits_to_f32(o) if ow == orc.F16 else o
     f64 = orc.lu_f64(a32 if ow == orc.F32 else orc.round_to(orc.F16, a32))
     assert info == 0
-    _check(g, o, f64, ow)
+    _check(g, o, f64, ow, f"rl W={W} I={I} O={O} nb={nb}")
 
 
 def test_c1_plain_fp32_ll_2048(handle, gpu):
@@ -129,6 +133,11 @@ def test_c1_plain_fp32_ll_2048(handle, gpu):
     a32 = orc.gen_diagdom_alpha(n, 1, 1.5)
     g, info = _run_gpu_helper(handle, gpu, "lu_nopiv_ll_f32", a32, nb)
     assert info == 0
+    # factor error on the GPU (tests/errors.hxx:251-346) against the oracle's
+    fe = rf.factor_error(handle, to_dev(a32, gpu), to_dev(g, gpu))
+    fo = orc.factor_error(a32, g)
+    assert fe["factor_cwerr"] == pytest.approx(fo, rel=1e-9)
+    assert fe["factor_cwerr"] < 64 * 2.0**-24
     L, Uu = np.tril(g.astype(np.float64), -1) + np.eye(n), np.triu(g.astype(np.float64))
     rel = np.linalg.norm(a32.astype(np.float64) - L @ Uu) / np.linalg.norm(a32.astype(np.float64))
     assert rel < 1e-6
@@ -140,7 +149,8 @@ def test_c1_plain_fp32_ll_2048(handle, gpu):
     xo = orc.getrs_f32(o, b.astype(np.float32))[:, 0].astype(np.float64)
     e_g, e_o = orc.backward_error(a32, b, x), orc.backward_error(a32, b, xo)
     assert e_g <= 4 * max(e_o, 2.0**-24)
-    _check(g, o, orc.lu_f64(a32), orc.F32)
+    assert rel == pytest.approx(fe["factor_rel_fro"], rel=1e-6)
+    _check(g, o, orc.lu_f64(a32), orc.F32, "C1 lu_nopiv_ll_f32 n=2048 nb=256")
 
 
 def test_info_reports_zero_pivot(handle, gpu):
@@ -181,10 +191,10 @@ def test_ll_bf16_outer_updates_match_oracle(handle, gpu, OP, n, nb, ib):
     da = to_dev(a16, gpu, ld=n + 3)
     assert rf.lu_nopiv(handle, da, nb, algo=rf.LL, cfg=cfg) == 0
     torch.cuda.synchronize()
-    g = orc.f16_bits_to_f32(to_np(da))
-    o = orc.f16_bits_to_f32(orc.lu_nopiv("ll", orc.cfg(W=orc.F16, I=orc.TC32, O=orc.BF16TC32,
-                                                        OP=op_or, ib=ib), a16, nb))
-    _check(g, o, orc.lu_f64(orc.round_to(orc.F16, a32)), orc.F16)
+    g = to_np(da)
+    o = orc.lu_nopiv("ll", orc.cfg(W=orc.F16, I=orc.TC32, O=orc.BF16TC32, OP=op_or, ib=ib),
+                     a16, nb)
+    _check(g, o, orc.lu_f64(orc.round_to(orc.F16, a32)), orc.F16, f"ll bf16 OP={OP} n={n}")
 
 
 @pytest.mark.parametrize("O", ["tc32", "bf16"])
@@ -212,31 +222,151 @@ def test_ll_lookahead_matches_oracle_and_sequential(handle, gpu, O):
             da = to_dev(a16, gpu, ld=n + 3)
             assert rf.lu_nopiv(handle, da, nb, algo=rf.LL, cfg=cfg) == 0
             torch.cuda.synchronize()
-            out[(on, rsv)] = orc.f16_bits_to_f32(to_np(da))
+            out[(on, rsv)] = to_np(da)
     finally:
         handle.set_lookahead(1, 0)
         handle.set_workspace(None)
-    o = orc.f16_bits_to_f32(orc.lu_nopiv("ll", orc.cfg(W=orc.F16, I=orc.TC32, O=oc_o,
-                                                        OP=orc.F32, ib=ib), a16, nb))
+    o = orc.lu_nopiv("ll", orc.cfg(W=orc.F16, I=orc.TC32, O=oc_o, OP=orc.F32, ib=ib), a16, nb)
     f64 = orc.lu_f64(orc.round_to(orc.F16, a32))
-    for f in out.values():
-        _check(f, o, f64, orc.F16)
+    for k, f in out.items():
+        _check(f, o, f64, orc.F16, f"lookahead {O} (on, reserve)={k}")
     # the masked and unmasked runs differ only in where the workgroups ran
     assert np.array_equal(out[(1, 32)], out[(1, 0)])
-    assert factor_distance(out[(1, 0)], out[(0, 32)]) <= 8 * U[orc.F16]
+    assert_parity(out[(1, 0)], out[(0, 32)], "lookahead on vs off")
 
 
 def test_set_lookahead_rejects_bad_arguments(handle, gpu):
-    """remifa_set_lookahead: on must be 0 / 1; reserve_cus a multiple of 8,
-    at most half the CUs (status -2 / -3, the knob unchanged)"""
+    """remifa_set_lookahead: on must be 0 / 1; reserve_cus a multiple of 8 in
+    [0, 64] (status -2 / -3); a rejected call leaves both settings as they
+    were (read back with remifa_get_lookahead), and a lookahead LU still
+    runs afterwards"""
+    import ctypes as C
+    import torch
     rf = _rf()
     from remifa_amd._lib import lib
     h = handle._h
-    assert lib().remifa_set_lookahead(h, 2, 0) == -2
-    assert lib().remifa_set_lookahead(h, 1, 12) == -3
-    assert lib().remifa_set_lookahead(h, 1, -8) == -3
-    assert lib().remifa_set_lookahead(h, 1, 4096) == -3
+    on, rsv = C.c_int(), C.c_int()
+    assert lib().remifa_set_lookahead(h, 1, 16) == 0
+    for bad_on, bad_rsv, st in ((2, 0, -2), (1, 12, -3), (1, -8, -3), (1, 72, -3), (1, 128, -3),
+                                (0, 4096, -3)):
+        assert lib().remifa_set_lookahead(h, bad_on, bad_rsv) == st
+        assert lib().remifa_get_lookahead(h, C.byref(on), C.byref(rsv)) == 0
+        assert (on.value, rsv.value) == (1, 16)
     assert lib().remifa_set_lookahead(None, 1, 0) == -1
+    assert lib().remifa_get_lookahead(None, None, None) == -1
     assert lib().remifa_set_lookahead(h, 1, 0) == 0
+    assert lib().remifa_get_lookahead(h, C.byref(on), C.byref(rsv)) == 0
+    assert (on.value, rsv.value) == (1, 0)
     with pytest.raises(rf.RemifaError):
         handle.set_lookahead(True, 7)
+    # the second stream is still used: the lookahead layout factors as before
+    n, nb = 384, 128
+    a16 = orc.f32_to_f16_bits(orc.gen_diagdom_alpha(n, 3, 1.2))
+    cfg = rf.api.make_config(W=rf.R_16F, I=rf.TC32, O=rf.TC32, OP=rf.R_32F, ib=32)
+    da = to_dev(a16, gpu)
+    assert rf.lu_nopiv(handle, da, nb, algo=rf.LL, cfg=cfg) == 0
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("O", ["tc32", "bf16", "fp8"])
+def test_headline_schedule_matches_oracle(handle, gpu, O):
+    """The headline configuration's schedule at n = 4096: fp16 storage, fp32
+    inner buffers and panels (src/lu_nopiv_ll.cxx:618), nb = 512, ib = 32,
+    lookahead on (eight outer blocks: seven second-stream updates, split-K
+    chosen by the cost model), TC32 / BF16 / FP8 outer updates; and the same
+    with split-K forced to 4 slices (the C-input reduction path). Both runs
+    against the oracle, and against each other."""
+    import torch
+    rf = _rf()
+    n, nb, ib = 4096, 512, 32
+    a32 = orc.gen_diagdom_alpha(n, 2024, 1.05)
+    a16 = orc.f32_to_f16_bits(a32)
+    oc_o, rf_o = {"tc32": (orc.TC32, rf.TC32), "bf16": (orc.BF16TC32, rf.BF16_TC32),
+                  "fp8": (orc.FP8TC32, rf.FP8_TC32)}[O]
+    cfg = rf.api.make_config(W=rf.R_16F, I=rf.TC32, O=rf_o, OP=rf.R_32F, ib=ib)
+    out = {}
+    try:
+        handle.set_lookahead(True, 0)
+        for split in (0, 4):
+            handle.set_tuning(0, split)
+            da = to_dev(a16, gpu)
+            assert rf.lu_nopiv(handle, da, nb, algo=rf.LL, cfg=cfg) == 0
+            torch.cuda.synchronize()
+            out[split] = to_np(da)
+    finally:
+        handle.set_tuning(0, 0)
+    o = orc.lu_nopiv("ll", orc.cfg(W=orc.F16, I=orc.TC32, O=oc_o, OP=orc.F32, ib=ib), a16, nb)
+    f64 = orc.lu_f64(orc.round_to(orc.F16, a32))
+    for split, f in out.items():
+        _check(f, o, f64, orc.F16, f"headline n={n} nb={nb} ib={ib} O={O} split={split}")
+    assert_parity(out[4], out[0], f"headline O={O} split 4 vs auto")
+
+
+@pytest.mark.parametrize("alpha", [1.01, 1.1, 2.0])
+def test_c2_fp32_storage_fp16_update_alpha_sweep(handle, gpu, alpha):
+    """Config 2 at n = 2048: fp32 storage with TC32 outer updates
+    (lu_nopiv_ll_f32_fp32xtc32, src/lu_nopiv_ll.cxx:74-87: I = FP32, O = TC32,
+    ib = 8), nb = 512, alpha in {1.01, 1.1, 2} (the diagonal dominance that
+    decides how far the fp16 update can be trusted). Factors against the
+    oracle; the reference harness's check solve (b = A 1, fp32 getrs with
+    identity pivots) on the GPU, with nwerr / cwerr computed on the GPU and
+    compared with the oracle's measures of the same solution; the factor
+    error on the GPU against the oracle's."""
+    import torch
+    rf = _rf()
+    n, nb = 2048, 512
+    a32 = orc.gen_diagdom_alpha(n, 7, alpha)
+    g, info = _run_gpu_helper(handle, gpu, "lu_nopiv_ll_f32_fp32xtc32", a32, nb)
+    assert info == 0
+    row = [r for r in rf.REFERENCE_HELPERS if r[0] == "lu_nopiv_ll_f32_fp32xtc32"][0]
+    o, _ = _oracle(row, a32, nb)
+    _check(g, o, orc.lu_f64(a32), orc.F32, f"C2 fp32xtc32 n={n} nb={nb} alpha={alpha}")
+    da = to_dev(a32, gpu)
+    dlu = to_dev(g, gpu)
+    b = a32.astype(np.float64) @ np.ones(n)  # tests/testing_lu_gpu.hxx:316-351
+    xb = to_dev(b.astype(np.float32), gpu)
+    rf.getrs_nopiv(handle, dlu, xb)
+    x = to_np(xb)[:, 0].astype(np.float64)
+    m = rf.error_measures(handle, da, to_dev(x, gpu), to_dev(b, gpu), lu=dlu)
+    assert m["nwerr"] == pytest.approx(orc.backward_error(a32, b, x), rel=1e-9)
+    cw, cwa = orc.compwise_error(a32, b, x, g)
+    assert m["cwerr"] == pytest.approx(cw, rel=1e-9)
+    assert m["cwerr_ax"] == pytest.approx(cwa, rel=1e-9)
+    # the solution from fp16-updated factors: backward error at the fp16 level
+    assert m["nwerr"] < 2.0**-11
+    fe = rf.factor_error(handle, da, dlu)
+    assert fe["factor_cwerr"] == pytest.approx(orc.factor_error(a32, g), rel=1e-9)
+
+
+@pytest.mark.parametrize("lu_dt,nrhs", [("f16", 1), ("f32", 1), ("f16", 3)])
+def test_error_measures_match_oracle(handle, gpu, lu_dt, nrhs):
+    """remifa_error_measures (nwerr, cwerr, Oettli-Prager) and
+    remifa_factor_error against the oracle's restatement of tests/errors.hxx
+    on the same data (n = 333: ragged chunks and tiles)."""
+    rf = _rf()
+    n = 333
+    a32 = orc.gen_diagdom_alpha(n, 5, 1.1)
+    if lu_dt == "f16":
+        lu_raw = orc.lu_nopiv("ll", orc.cfg(W=orc.F16, I=orc.TC32, O=orc.TC32, OP=orc.F32,
+                                            ib=32), orc.f32_to_f16_bits(a32), 64)
+    else:
+        lu_raw = orc.lu_nopiv("ll", orc.cfg(), a32, 64)
+    lu = _f(lu_raw)
+    rng = np.random.default_rng(3)
+    b = rng.uniform(-1, 1, (n, nrhs))
+    x = np.asfortranarray(orc.getrs_f64(lu.astype(np.float64), b))
+    da, dlu = to_dev(a32, gpu), to_dev(lu_raw, gpu)
+    m = rf.error_measures(handle, da, to_dev(x, gpu), to_dev(np.asfortranarray(b), gpu), lu=dlu)
+    assert m["nwerr"] == pytest.approx(orc.backward_error(a32, b, x), rel=1e-9)
+    cws = [orc.compwise_error(a32, b[:, k], x[:, k], lu) for k in range(nrhs)]
+    assert m["cwerr"] == pytest.approx(max(c[0] for c in cws), rel=1e-9)
+    assert m["cwerr_ax"] == pytest.approx(max(c[1] for c in cws), rel=1e-9)
+    m0 = rf.error_measures(handle, da, to_dev(x, gpu), to_dev(np.asfortranarray(b), gpu))
+    assert np.isnan(m0["cwerr"]) and m0["nwerr"] == m["nwerr"]
+    fe = rf.factor_error(handle, da, dlu)
+    assert fe["factor_cwerr"] == pytest.approx(orc.factor_error(a32, lu), rel=1e-9)
+    L, Uu = np.tril(lu.astype(np.float64), -1) + np.eye(n), np.triu(lu.astype(np.float64))
+    e = a32.astype(np.float64) - L @ Uu
+    assert fe["factor_abs"] == pytest.approx(np.abs(e).max(), rel=1e-9)
+    assert fe["factor_rel_fro"] == pytest.approx(
+        np.linalg.norm(e) / np.linalg.norm(a32.astype(np.float64)), rel=1e-9)

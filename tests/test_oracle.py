@@ -8,6 +8,7 @@ dropped term, a wrong index or sign, a division where the reference
 multiplies by a reciprocal, unfused where it fuses) fail."""
 import json
 import os
+import sys
 from fractions import Fraction
 
 import numpy as np
@@ -506,6 +507,85 @@ def test_factor_error_zero_for_exact_factors():
     assert orc.factor_error(a, lu2) > 0.0
 
 
+def test_factor_error_hand_computed():
+    """max_ij |A - LU|_ij / (|L||U|)_ij on cases worked by hand
+    (tests/errors.hxx:251-346). 2x2: A = [[4, 2], [2, 3]] with l21 = 3/4,
+    u22 = 9/4: |A - LU| = [[0, 0], [1, 3/4]], |L||U| = [[4, 2], [3, 15/4]], so
+    the error is 1/3 (dividing by |A| would give 1/2). 3x3 with mixed signs,
+    L = [[1,0,0],[-1/2,1,0],[1/4,1,1]], U = [[2,-1,1],[0,3,-2],[0,0,4]]: LU =
+    [[2,-1,1],[-1,7/2,-5/2],[1/2,11/4,9/4]], |L||U| = [[2,1,1],[1,7/2,5/2],
+    [1/2,13/4,25/4]]; A = LU except a_21 += 1/2 and a_12 -= 1/4, so the
+    error is max(1/2 / 13/4, 1/4 / 5/2) = 2/13 (|LU| in the denominator would
+    give 2/11)."""
+    a = np.array([[4, 2], [2, 3]], np.float32)
+    lu = np.array([[4, 2], [0.75, 2.25]], np.float32)
+    assert orc.factor_error(a, lu) == pytest.approx(1.0 / 3.0, rel=1e-15)
+    lu3 = np.array([[2, -1, 1], [-0.5, 3, -2], [0.25, 1, 4]], np.float32)
+    a3 = np.array([[2, -1, 1], [-1, 3.5, -2.75], [0.5, 3.25, 2.25]], np.float32)
+    assert orc.factor_error(a3, lu3) == pytest.approx(2.0 / 13.0, rel=1e-15)
+
+
+# ---------------------------------------------------------------- parity rule
+def _ll16(a16, nb, **kw):
+    c = dict(W=orc.F16, I=orc.TC32, O=orc.TC32, OP=orc.F32, ib=32)
+    c.update(kw)
+    return orc.lu_nopiv("ll", orc.cfg(**c), a16, nb)
+
+
+@pytest.mark.parametrize("n,nb,alpha", [(256, 64, 1.5), (700, 128, 1.1)])
+def test_parity_rule_rejects_planted_rounding_mistakes(n, nb, alpha):
+    """The entrywise rule the GPU parity tests apply (tests/helpers.py)
+    accepts two oracle results that differ only in summation order (k
+    ascending vs descending: what the matrix cores change) and rejects
+    factors computed with one rounding point moved, planted here in the
+    oracle itself: fp16 outer panel buffers where the headline layout keeps
+    fp32 ones, bf16 instead of fp16 outer operands, unrounded (fp32) instead
+    of fp16 inner operands; for fp32 storage, fp16-rounded outer operands
+    where fp32 ones are used and the reverse."""
+    from helpers import parity_ok
+    a32 = orc.gen_diagdom_alpha(n, 11, alpha)
+    a16 = orc.f32_to_f16_bits(a32)
+    ref = _ll16(a16, nb)
+    f32 = lambda I, O: orc.lu_nopiv("ll", orc.cfg(W=orc.F32, I=I, O=O, ib=8), a32, nb)
+    r32 = f32(orc.FP32, orc.TC32)
+    try:
+        orc.set_sum_order(1)
+        rev16, rev32 = _ll16(a16, nb), f32(orc.FP32, orc.TC32)
+    finally:
+        orc.set_sum_order(0)
+    assert not np.array_equal(rev32, r32)  # the order does change the results
+    for g, o in ((ref, ref), (rev16, ref), (r32, r32), (rev32, r32)):
+        ok, st = parity_ok(g, o)
+        assert ok, st
+    for wrong in (dict(OP=orc.F16, IP=orc.F32, OF=orc.F16, IF=orc.F32), dict(O=orc.BF16TC32),
+                  dict(I=orc.FP32)):
+        ok, st = parity_ok(_ll16(a16, nb, **wrong), ref)
+        assert not ok, (wrong, st)
+    for I, O in ((orc.FP32, orc.FP32), (orc.TC32, orc.TC32)):
+        ok, st = parity_ok(f32(I, O), r32)
+        assert not ok, (I, O, st)
+
+
+def test_entry_scale_closed_form():
+    """s = |L||U| (divided by |u_jj| below the diagonal) on a 2x2 case:
+    L = [[1, 0], [-1/2, 1]], U = [[4, -2], [0, 3]]: |L||U| = [[4, 2], [2, 4]],
+    so s = [[4, 2], [2 / 4, 4]]"""
+    from helpers import entry_scale
+    lu = np.array([[4, -2], [-0.5, 3]], np.float32)
+    assert np.allclose(entry_scale(lu), [[4, 2], [0.5, 4]], rtol=0, atol=0)
+
+
+def test_ulp_distance_known_values():
+    from helpers import ulp_distance
+    h = orc.f32_to_f16_bits(np.array([1.0, 1.0, -0.0, 65504.0, -1.0], np.float32))
+    h2 = orc.f32_to_f16_bits(np.array([1.0 + 2**-10, 1.0 - 2**-11, 0.0, 32768.0, 1.0],
+                                      np.float32))
+    assert ulp_distance(h, h2).ravel().tolist() == [1, 1, 0, 0x7BFF - 0x7800, 2 * 0x3C00]
+    f = np.array([1.0, -0.0, 2.0], np.float32)
+    f2 = np.array([np.nextafter(np.float32(1), np.float32(2)), 0.0, 2.0], np.float32)
+    assert ulp_distance(f, f2).ravel().tolist() == [1, 0, 0]
+
+
 # ---------------------------------------------------------------- residual / IR
 def test_residual_f64_vs_exact():
     rng = np.random.default_rng(12)
@@ -548,3 +628,87 @@ def test_ir_trace_fixture():
     assert orc.backward_error(a, b, x) <= 1e-14
     # refinement contracts by about the fp16 factorization's error each step
     assert all(h1 < h0 * 1e-2 for h0, h1 in zip(hist[:-1], hist[1:]))
+
+
+# ---------------------------------------------------------------- generators / equilibration
+def _fnv1a(a: np.ndarray) -> str:
+    h = 1469598103934665603
+    for b in np.asfortranarray(a, np.float32).ravel(order="F").view(np.uint8):
+        h ^= int(b)
+        h = (h * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return f"{h:016x}"
+
+
+def test_arrowhead_and_randmat_match_libstdcxx():
+    """gen_arrowhead / gen_randmat (tests/common.hxx:298-319, 391-455) of the
+    oracle against the same loops run with libstdc++'s engine and
+    uniform_real_distribution<float> (tests/golden/minstd_ref.cpp)"""
+    g = json.load(open(os.path.join(GOLDEN, "generators.json")))
+    for n, lo, hi, h in g["arrowhead"]:
+        assert _fnv1a(orc.gen_arrowhead(n, lo, hi)) == h, (n, lo, hi)
+    for m, n, seed, h in g["randmat"]:
+        assert _fnv1a(orc.gen_randmat(m, n, seed)) == h, (m, n, seed)
+
+
+def test_arrowhead_structure():
+    a = orc.gen_arrowhead(600, -1.0, 1.0)
+    assert np.all(np.diag(a) == 512.0)
+    nz = a != 0
+    assert not nz[0:256, 256:344].any() and not nz[256:344, 0:256].any()  # off-block zeros
+    assert nz[344:, :].mean() > 0.99 and nz[:, 344:].mean() > 0.99      # the arrow
+    assert np.all(np.abs(a[a != 512.0]) < 1.0)
+
+
+def test_geequb_matches_lapack():
+    """the oracle's restatement of sgeequb against scipy's bundled LAPACK on
+    badly scaled matrices (tests/golden/geequb.json): bit-identical scale
+    factors, same rowcnd / colcnd / amax / info"""
+    g = json.load(open(os.path.join(GOLDEN, "geequb.json")))
+    for c in g["cases"]:
+        m, n = c["m"], c["n"]
+        a = np.array(c["a_bits"], np.uint32).view(np.float32).reshape((m, n), order="F")
+        r, cc, rowcnd, colcnd, amax, info = orc.geequb(a)
+        assert info == c["info"]
+        assert r.view(np.uint32).tolist() == c["r_bits"]
+        assert cc.view(np.uint32).tolist() == c["c_bits"]
+        assert (rowcnd, colcnd, amax) == (np.float32(c["rowcnd"]), np.float32(c["colcnd"]),
+                                          np.float32(c["amax"]))
+
+
+def test_geequb_zero_row_and_column_info():
+    a = np.asfortranarray(np.eye(4, dtype=np.float32))
+    a[2, 2] = 0.0
+    assert orc.geequb(a)[5] == 3
+    b = np.asfortranarray(np.ones((3, 3), np.float32))
+    b[:, 1] = 0.0
+    assert orc.geequb(b)[5] == 3 + 2
+
+
+def test_laqge_branches_closed_form():
+    """slaqge: no scaling when rowcnd and colcnd >= 0.1 (and amax in range),
+    columns only when just colcnd < 0.1, rows only when just rowcnd < 0.1,
+    both otherwise; power-of-two factors multiply exactly"""
+    a = np.asfortranarray(np.array([[3.0, 5.0], [7.0, 11.0]], np.float32))
+    r = np.array([0.5, 0.25], np.float32)
+    c = np.array([2.0, 0.125], np.float32)
+    out, eq = orc.laqge(a, r, c, 1.0, 1.0, 11.0)
+    assert eq == "N" and np.array_equal(out, a)
+    out, eq = orc.laqge(a, r, c, 1.0, 0.05, 11.0)
+    assert eq == "C" and np.array_equal(out, a * c[None, :])
+    out, eq = orc.laqge(a, r, c, 0.05, 1.0, 11.0)
+    assert eq == "R" and np.array_equal(out, a * r[:, None])
+    out, eq = orc.laqge(a, r, c, 0.05, 0.05, 11.0)
+    assert eq == "B" and np.array_equal(out, a * r[:, None] * c[None, :])
+    out, eq = orc.laqge(a, r, c, 1.0, 1.0, 2.0**-110)  # amax below SMALL: rows scaled
+    assert eq == "R"
+
+
+def test_read_mtx_symmetric_fixture():
+    """the harness's MatrixMarket reader (tests/genmat.hxx:16-63): every
+    stored entry lands at (row, col) and (col, row)"""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from testing_lu_gpu import read_mtx
+    a, nrows = read_mtx(os.path.join(GOLDEN, "small_sym.mtx"))
+    want = np.array([[4, -1.5, 0, 0, 2], [-1.5, 5.25, 0, 0.125, 0], [0, 0, 6, 0, 0],
+                     [0, 0.125, 0, 7, 0], [2, 0, 0, 0, 8.5]], np.float32)
+    assert nrows == 5 and np.array_equal(a, want)
