@@ -243,6 +243,18 @@ int remifa_gemm_fp8(remifa_handle_t h, int out_t, int64_t m, int64_t n, int64_t 
 int remifa_getrs_nopiv(remifa_handle_t h, int lu_t, int trans, int64_t n, int64_t nrhs,
                        const void *d_lu, int64_t ldlu, float *d_b, int64_t ldb);
 
+/* remifa_getrs_nopiv without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture
+ * setup, serving loops, harnesses that forbid hipMalloc). Same arguments,
+ * kernels and results as remifa_getrs_nopiv, but its scratch (sync words,
+ * inverted diagonal blocks, tagged solution words: about 36 MiB) is a
+ * static device array that every GPU the library runs on holds until the
+ * process ends, instead of the handle's workspace. n is limited to 32768
+ * (REMIFA_ERR_WORKSPACE above it). The array is shared by every call on a
+ * GPU: calls that overlap on one GPU (other streams, other threads) race. */
+int remifa_getrs_nopiv_noalloc(remifa_handle_t h, int lu_t, int trans, int64_t n, int64_t nrhs,
+                               const void *d_lu, int64_t ldlu, float *d_b, int64_t ldb);
+
 /* FP64 residual R = B - A X (A fp32 n x n, X/B/R fp64 n x nrhs). No
  * reference counterpart (the reference checks on the host in FP64,
  * tests/errors.hxx:186-249). */

@@ -25,7 +25,7 @@ from .api import (  # noqa: F401
     Handle, colmajor_empty, to_colmajor, dtype_code,
     lu_nopiv, lu_nopiv_panel, convert, gemmex, quantize_fp8, gemm_fp8, getrs_nopiv, residual_f64,
     ir_solve, error_measures, factor_error, gen_randmat, gen_arrowhead, geequb, laqge,
-    conv_error,
+    conv_error, getrs_nopiv_noalloc,
     gen_diagdom, gen_uniform, lu_workspace_size, lu_nopiv_batched, getrs_nopiv_batched,
     REFERENCE_HELPERS,
 )

@@ -76,6 +76,7 @@ SIGNATURES = {
     "remifa_gemm_fp8": [_vp, _i, _i64, _i64, _i64, _f, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64,
                         _f, _vp, _i64],
     "remifa_getrs_nopiv": [_vp, _i, _i, _i64, _i64, _vp, _i64, _vp, _i64],
+    "remifa_getrs_nopiv_noalloc": [_vp, _i, _i, _i64, _i64, _vp, _i64, _vp, _i64],
     "remifa_residual_f64": [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64],
     "remifa_ir_solve": [_vp, _i, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _d, _i,
                         _pi, _pd, _vp],

@@ -285,6 +285,16 @@ def gemm_fp8(handle, qa: torch.Tensor, sa: torch.Tensor, qb: torch.Tensor, sb: t
 
 
 # ------------------------------------------------------------------ solves
+def getrs_nopiv_noalloc(handle, lu: torch.Tensor, b: torch.Tensor, trans: bool = False):
+    """getrs_nopiv with its scratch in a static device array (no allocation;
+    n <= 32768; calls that overlap on one GPU race)."""
+    n = lu.shape[0]
+    nrhs = b.shape[1] if b.dim() == 2 else 1
+    check(lib().remifa_getrs_nopiv_noalloc(_h(handle), dtype_code(lu), int(trans), n, nrhs,
+                                           _p(lu), _ld(lu), _p(b), _ld(b)),
+          "remifa_getrs_nopiv_noalloc")
+
+
 def getrs_nopiv(handle, lu: torch.Tensor, b: torch.Tensor, trans: bool = False):
     """b (fp32, n x nrhs column-major) <- (LU)^-1 b or (LU)^-T b."""
     n = lu.shape[0]

@@ -84,6 +84,9 @@ struct remifa_handle_st {
   // autotuned GEMM rules, first match wins: {opk (-1 any), m_lo, m_hi, n_lo,
   // n_hi, tile (-1 heuristic), split (0 heuristic)}, bounds inclusive
   std::vector<std::array<int64_t, 7>> gemm_rules;
+  // device address of this device's static scratch array of the
+  // no-allocation entry point (solve.hip), looked up at creation
+  void* scratch = nullptr;
 };
 
 namespace remifa {
@@ -248,5 +251,9 @@ int fp8_gemm_launch(remifa_handle_t h, int out_t, int64_t m, int64_t n, int64_t 
 
 int copy2d_launch(remifa_handle_t h, size_t elt, int64_t m, int64_t n, const void* a, int64_t lda,
                   void* b, int64_t ldb);
+
+// device address of the static scratch array of remifa_getrs_nopiv_noalloc
+// on `device` (current), looked up once per device (solve.hip)
+int noalloc_scratch(int device, void** ptr);
 
 }  // namespace remifa

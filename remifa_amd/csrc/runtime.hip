@@ -84,6 +84,9 @@ int remifa_create(remifa_handle_t* handle, void* stream) {
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, h->device) == hipSuccess) h->num_cu = prop.multiProcessorCount;
+  // the no-allocation solve's scratch on this device (a failure is reported
+  // by that entry point, not here)
+  if (noalloc_scratch(h->device, &h->scratch) != 0) h->scratch = nullptr;
   *handle = h;
   return 0;
 }

@@ -21,6 +21,8 @@
 // chunk), fp64 fma per element, per-chunk partials summed in chunk order by a
 // second kernel (deterministic).
 #include <algorithm>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -654,6 +656,93 @@ int remifa_getrs_nopiv(remifa_handle_t h, int lu_t, int trans, int64_t n, int64_
   const TrsmWs w = trsm_ws(h->ws, n);
   RF_CHECK(trsm_invert_launch(h, lu_t, n, d_lu, ldlu, w));
   RF_CHECK(getrs_launch(h, lu_t, trans, n, nrhs, d_lu, ldlu, d_b, ldb, w));
+  int err = 0;
+  RF_HIP(hipMemcpyAsync(&err, w.sync + 1, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  RF_HIP(hipStreamSynchronize(h->stream));
+  if (err) {
+    set_last_error("triangular solve: dependency wait timed out");
+    return REMIFA_ERR_HIP;
+  }
+  return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- no-allocation solve
+// Scratch of remifa_getrs_nopiv_noalloc: a file-scope device array sized for
+// the largest order it takes (sync words, the inverted 128 x 128 diagonal
+// blocks of L and U, the n x 16 tagged solution words: trsm_ws_bytes(n)).
+// Every GPU holds its own copy for the life of the process. Host code never
+// uses the array's name as an address: its device address comes from
+// hipGetSymbolAddress, once per device (noalloc_scratch).
+namespace remifa {
+namespace {
+constexpr int64_t kNoAllocMaxN = 32768;
+constexpr size_t kNoAllocBytes = 256 + 2 * sizeof(float) * NBS * NBS * (kNoAllocMaxN / NBS) +
+                                 sizeof(unsigned long long) * kNoAllocMaxN * RC;
+static_assert(kNoAllocMaxN % NBS == 0, "whole diagonal blocks");
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// device address of this device's scratch array (looked up on the first call
+// for the device, which must be current)
+int noalloc_scratch(int device, void** ptr) {
+  *ptr = nullptr;
+  if (device < 0 || device >= kMaxDevices) {
+    set_last_error("device index out of range of the scratch table");
+    return REMIFA_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    RF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch[device] = p;
+  }
+  *ptr = g_scratch[device];
+  return 0;
+}
+
+size_t noalloc_scratch_bytes() { return kNoAllocBytes; }
+}  // namespace remifa
+
+extern "C" {
+
+int remifa_getrs_nopiv_noalloc(remifa_handle_t h, int lu_t, int trans, int64_t n, int64_t nrhs,
+                               const void* d_lu, int64_t ldlu, float* d_b, int64_t ldb) {
+  if (!h) return -1;
+  if (lu_t != REMIFA_R_32F && lu_t != REMIFA_R_16F) return -2;
+  if (n < 0) return -4;
+  if (nrhs < 0) return -5;
+  if (!d_lu && n > 0) return -6;
+  if (ldlu < std::max<int64_t>(n, 1)) return -7;
+  if (!d_b && n > 0 && nrhs > 0) return -8;
+  if (ldb < std::max<int64_t>(n, 1)) return -9;
+  if (n > kNoAllocMaxN) {
+    set_last_error("remifa_getrs_nopiv_noalloc: n = " + std::to_string(n) +
+                   " exceeds the largest order of its static scratch, " +
+                   std::to_string(kNoAllocMaxN));
+    return REMIFA_ERR_WORKSPACE;
+  }
+  if (n == 0 || nrhs == 0) return 0;
+  if (!h->scratch) {
+    set_last_error("remifa_getrs_nopiv_noalloc: no scratch address for this handle's device");
+    return REMIFA_ERR_HIP;
+  }
+  // the kernels of remifa_getrs_nopiv on the static scratch; the timer is
+  // held off (its event pool would grow), the sync words and solution tags
+  // are cleared by fill kernels inside getrs_launch, the inverted blocks
+  // are rewritten, so nothing of an earlier call's contents is read
+  const bool timed = h->timer.enabled;
+  h->timer.enabled = false;
+  const TrsmWs w = trsm_ws(h->scratch, n);
+  int st = trsm_invert_launch(h, lu_t, n, d_lu, ldlu, w);
+  if (!st) st = getrs_launch(h, lu_t, trans, n, nrhs, d_lu, ldlu, d_b, ldb, w);
+  h->timer.enabled = timed;
+  RF_CHECK(st);
   int err = 0;
   RF_HIP(hipMemcpyAsync(&err, w.sync + 1, sizeof(int), hipMemcpyDeviceToHost, h->stream));
   RF_HIP(hipStreamSynchronize(h->stream));
