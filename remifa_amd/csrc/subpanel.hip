@@ -475,6 +475,132 @@ __global__ void __launch_bounds__(VL) vec_kernel(int64_t cm, int sw, int ib, T* 
   }
 }
 
+// 2b. full 64-wide slices: the same recurrence, one ib block of the vector
+// at a time in registers (the finished blocks kept op-rounded for the
+// cross-block sums), the tile in LDS read as broadcasts. Each entry receives
+// the same operations in the same order as in vec_kernel: the cross-block
+// sums (LL: one sum over every finished block, RL: one per finished block,
+// in block order), then the pivots of its own block in pivot order. A
+// workgroup of 256 lanes takes one role (rows of L or columns of U).
+constexpr int VS = 256;  // lanes per workgroup
+
+template <typename T, int MODE, bool RL, int IB>
+__global__ void __launch_bounds__(VS) vecs_kernel(int64_t cm, T* __restrict__ l, int64_t ldl,
+                                                  T* __restrict__ u, int64_t ldu,
+                                                  half_t* __restrict__ cp, int64_t ldcp,
+                                                  const float* __restrict__ tab, int nl_wg,
+                                                  int vflags) {
+  using Sm = Sem<MODE>;
+  constexpr bool OPR = MODE == SM_F32_TC || MODE == SM_F16_BF;  // op(M) differs from M
+  constexpr bool XSUM = RL || IB < SW;                          // cross-block sums occur
+  __shared__ __attribute__((aligned(16))) float Ms[SW * SW];
+  __shared__ __attribute__((aligned(16))) float Mos[(OPR && XSUM) ? SW * SW : 4];
+  __shared__ float Rs[SW];
+  const int t = threadIdx.x;
+  const int wg = blockIdx.x;
+  const bool is_l = wg < nl_wg;  // workgroup-uniform
+  {
+    const float4* M4 = (const float4*)(tab + (is_l ? T_D : T_DT));
+    const float4* Mo4 = (const float4*)(tab + (is_l ? T_DOP : T_DOPT));
+#pragma unroll
+    for (int e = t; e < SW * SW / 4; e += VS) {
+      ((float4*)Ms)[e] = M4[e];
+      if constexpr (OPR && XSUM) ((float4*)Mos)[e] = Mo4[e];
+    }
+    if (t < SW) Rs[t] = tab[(is_l ? T_R : T_ONE) + t];
+  }
+  const float* const Mo = (OPR && XSUM) ? Mos : Ms;
+  __syncthreads();
+  const int64_t idx = SW + (int64_t)(is_l ? wg : wg - nl_wg) * VS + t;
+  if (idx >= cm) return;  // no barriers below
+  T* const p = is_l ? l + idx : u + idx * ldu;
+  const int64_t st = is_l ? ldl : 1;
+  const bool vp = !is_l && (vflags & 1);
+  half_t* const q = cp ? (is_l ? cp + idx : cp + idx * ldcp) : nullptr;
+  const int64_t sq = is_l ? ldcp : 1;
+  const bool vq = !is_l && (vflags & 2);
+  constexpr int EV = 16 / sizeof(T);
+  float xo[XSUM ? SW : 1];  // op-rounded finished entries
+#pragma unroll
+  for (int b0 = 0; b0 < SW; b0 += IB) {
+    asm volatile("" ::: "memory");  // one block at a time
+    float x[IB];
+    if (vp) {
+#pragma unroll
+      for (int j = 0; j < IB; j += EV) ld16<T>(p + b0 + j, x + j);
+    } else {
+#pragma unroll
+      for (int j = 0; j < IB; ++j) x[j] = (float)p[(b0 + j) * st];
+    }
+    if constexpr (XSUM) {
+      // cross-block sums: LL one over every finished block, RL one per block
+#pragma unroll
+      for (int pb = 0; pb < b0; pb += (RL ? IB : SW)) {
+        const int pe = RL ? pb + IB : b0;
+        float acc[IB];
+#pragma unroll
+        for (int c = 0; c < IB; ++c) acc[c] = 0.0f;
+#pragma unroll
+        for (int j = pb; j < pe; ++j) {
+#pragma unroll
+          for (int c = 0; c < IB; c += 4) {
+            const float4 m = *(const float4*)&Mo[j * SW + b0 + c];
+            acc[c] = Sm::acc_add(acc[c], xo[j], m.x);
+            acc[c + 1] = Sm::acc_add(acc[c + 1], xo[j], m.y);
+            acc[c + 2] = Sm::acc_add(acc[c + 2], xo[j], m.z);
+            acc[c + 3] = Sm::acc_add(acc[c + 3], xo[j], m.w);
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < IB; ++c) x[c] = Sm::apply(x[c], acc[c]);
+      }
+    }
+    // this block's pivots, in order, groups of 8 entries
+#pragma unroll
+    for (int g0 = 0; g0 < IB; g0 += G) {
+#pragma unroll
+      for (int k = 0; k < g0; ++k) {
+        const float4 a = *(const float4*)&Ms[(b0 + k) * SW + b0 + g0];
+        const float4 b = *(const float4*)&Ms[(b0 + k) * SW + b0 + g0 + 4];
+        const float mr[G] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int c = 0; c < G; ++c) x[g0 + c] = Sm::fnma(x[k], mr[c], x[g0 + c]);
+      }
+#pragma unroll
+      for (int kk = 0; kk < G; ++kk) {
+        const float lk = Sm::mul(x[g0 + kk], Rs[b0 + g0 + kk]);
+        x[g0 + kk] = lk;
+        const float4 a = *(const float4*)&Ms[(b0 + g0 + kk) * SW + b0 + g0];
+        const float4 b = *(const float4*)&Ms[(b0 + g0 + kk) * SW + b0 + g0 + 4];
+        const float mr[G] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int c = kk + 1; c < G; ++c) x[g0 + c] = Sm::fnma(lk, mr[c], x[g0 + c]);
+      }
+    }
+    // the finished block: factors, mirror, op-rounded copy
+    if (vp) {
+#pragma unroll
+      for (int j = 0; j < IB; j += EV) st16<T>(p + b0 + j, x + j);
+    } else {
+#pragma unroll
+      for (int j = 0; j < IB; ++j) p[(b0 + j) * st] = (T)x[j];
+    }
+    if (q) {
+      if (vq) {
+#pragma unroll
+        for (int j = 0; j < IB; j += 8) st16<half_t>(q + b0 + j, x + j);
+      } else {
+#pragma unroll
+        for (int j = 0; j < IB; ++j) q[(b0 + j) * sq] = (half_t)x[j];
+      }
+    }
+    if constexpr (XSUM) {
+#pragma unroll
+      for (int j = 0; j < IB; ++j) xo[b0 + j] = Sm::op(x[j]);
+    }
+  }
+}
+
 template <typename T, int MODE, bool RL>
 int launch_typed(remifa_handle_t h, int ib, int64_t cm, int sw, void* l, int64_t ldl, void* u,
                  int64_t ldu, half_t* cp, int64_t ldcp, float* tab, double bytes) {
@@ -489,8 +615,23 @@ int launch_typed(remifa_handle_t h, int ib, int64_t cm, int sw, void* l, int64_t
   const int vflags = ((uintptr_t)u % 16 == 0 && (ldu * sizeof(T)) % 16 == 0 ? 1 : 0) |
                      (cp && (uintptr_t)cp % 16 == 0 && (ldcp * 2) % 16 == 0 ? 2 : 0);
   timer_begin(h, KC_PANEL, bytes, 1);
-  vec_kernel<T, MODE, RL><<<2 * nl, VL, 0, h->stream>>>(cm, sw, ib, (T*)l, ldl, (T*)u, ldu, cp,
-                                                         ldcp, tab, nl, vflags);
+  if (sw == SW) {
+    // full slices: one ib block of each vector at a time in registers
+    const int nw = (int)ceil_div(rest, VS);
+#define RF_VECS(IBV)                                                                          \
+  vecs_kernel<T, MODE, RL, IBV><<<2 * nw, VS, 0, h->stream>>>(cm, (T*)l, ldl, (T*)u, ldu, cp, \
+                                                               ldcp, tab, nw, vflags)
+    switch (ib) {
+      case 8: RF_VECS(8); break;
+      case 16: RF_VECS(16); break;
+      case 32: RF_VECS(32); break;
+      default: RF_VECS(64); break;
+    }
+#undef RF_VECS
+  } else {
+    vec_kernel<T, MODE, RL><<<2 * nl, VL, 0, h->stream>>>(cm, sw, ib, (T*)l, ldl, (T*)u, ldu, cp,
+                                                           ldcp, tab, nl, vflags);
+  }
   RF_HIP(hipGetLastError());
   timer_end(h);
   return 0;
