@@ -1,0 +1,487 @@
+// Inner factorization of one outer block of the left-looking driver in two
+// launches: the reference's ib loop (src/lu_nopiv_inner_ll.hxx:197-370)
+// with ib = 32, for the buffered layout (fp16 storage, the block
+// column and block row already updated into fp32 buffers, TC32 inner
+// updates, fp32 panel arithmetic).
+//
+// The nb x nb diagonal block is cut into slices of 64 columns (two ib
+// blocks). A "vector" is a row of L below a slice's 64 x 64 diagonal tile
+// or a column of U right of it; a "unit" is 32 consecutive vectors of one
+// kind, the N side of one MFMA tile. For a unit and a slice at column so,
+// with M the tile's U (rows of L) or L^T (columns of U):
+//   block 0 (entries so .. so+31):  x = C - sum_{k < so}      op(.) op(.)
+//   block 1 (entries so+32 .. +63): x = C - sum_{k < so + 32} op(.) op(.)
+// each one fp32 accumulation on the matrix cores (v_mfma_f32_32x32x16_f16:
+// op = fp16 rounding) that starts from -C, as the drivers' update GEMMs
+// start from C, then the block's pivots in order: x_k *= 1/u_kk (rows of
+// L), x_e -= x_k M[k][e] (fp32 fused multiply-add, entry e in pivot
+// order). That is the
+// reference's per-ib-block GEMM (one update over every column of the outer
+// block left of the ib block) followed by its panel kernel
+// (src/lu_nopiv.cu:123-170, 251-326); the slice kernels (subpanel.hip)
+// split the same update sum at the slice boundary instead.
+//
+// Two launches per outer block, on the handle's stream:
+//  * the diagonal kernel, one workgroup: for each slice, the tile's update
+//    (MFMA, K = so), its factorization (tile_factor, eight wavefronts),
+//    its slice table (the factored tile in the layouts the units read)
+//    written to global memory, then the units inside the diagonal block
+//    (rows / columns so+64 .. nb-1) for that slice;
+//  * the far kernel: the units of the rows of L and columns of U beyond the
+//    diagonal block, each through every slice (a workgroup stages a slice's
+//    table and the other factor's tile vectors in LDS, then its wavefronts
+//    take their units). Its workgroups are small enough (registers, LDS) to
+//    share a CU with the second stream's update GEMM.
+// A far unit reads the slice tables, the other factor's vectors inside the
+// diagonal block and its own earlier results: everything it reads was
+// written by the diagonal kernel before the launch or by itself.
+//
+// MFMA operand layout used below (v_mfma_f32_32x32x16_f16, D = A B + C):
+// lane l supplies A[i = l % 32][k = 8 (l / 32) .. +7] and
+// B[k = 8 (l / 32) .. +7][j = l % 32]; register r of lane l holds
+// D[i = 8 (r / 4) + 4 (l / 32) + r % 4][j = l % 32]. With A = the other
+// factor (rows i = tile entries) and B = the unit's own vectors (j = vector),
+// lane (n, h) holds 16 of the 32 entries of vector n in a block:
+// e = 8 q + 4 h + j for register 4 q + j. A pivot's multiplier goes to the
+// other half with v_permlane32_swap (lanes 0-31 <-> 32-63).
+#include <algorithm>
+
+#include "slice_common.hpp"
+
+namespace remifa {
+namespace {
+
+constexpr int BLT = 512;  // threads per workgroup (eight wavefronts)
+constexpr int NV = 32;    // vectors per unit
+constexpr int NBM = 512;  // largest outer block
+constexpr int KSM = (NBM - SW) / 16;  // most k-steps of a unit's update
+constexpr int PP = NBM - SW + 8;      // row pitch (halves) of the staged panel
+constexpr int kBufWord3 = 0x00020000;  // buffer resource word 3 (raw 32-bit data)
+
+typedef _Float16 bh8 __attribute__((ext_vector_type(8)));
+typedef _Float16 bh4 __attribute__((ext_vector_type(4)));
+typedef float bf16v __attribute__((ext_vector_type(16)));
+
+// one role's part of a slice table (rows of L: M = D; columns of U: M = D^T)
+struct RoleTab {
+  // the diagonal 32 x 32 blocks, m[b][k][e] = M[32 b + k][32 b + e] for
+  // e != k; the diagonal holds the pivot's scale instead (1 / u_kk for the
+  // rows of L, 1 for the columns of U), so that a pivot's row read brings it
+  float m[2][32][32];
+  _Float16 mot[32][32];   // op(M[k][32 + e]) stored [e][k] (cross-block operand)
+};
+struct SliceTab {
+  RoleTab role[2];
+};
+static_assert(sizeof(RoleTab) % 16 == 0, "role table size");
+
+struct BlkArgs {
+  int64_t updm;         // rows of the block column = columns of the block row
+  int nb;               // width of the outer block (multiple of 64, <= NBM)
+  int nfu;              // far units of each role
+  int nfw;              // far workgroups of each role
+  const float* pl;      // block column buffer (updm x nb, fp32)
+  int64_t ldpl;
+  const float* pu;      // block row buffer (nb x updm, fp32)
+  int64_t ldpu;
+  half_t* a;            // the factors (fp16), the outer block at (0, 0)
+  int64_t lda;
+  half_t* lop;          // op(L) rows, k-contiguous: lop[v * ldo + k]
+  half_t* uop;          // op(U) columns, k-contiguous: uop[v * ldo + k]
+  int64_t ldo;
+  SliceTab* tabs;       // one per slice
+};
+
+// LDS of the diagonal kernel (one workgroup; 162,048 bytes of the CU's 160 KiB)
+struct DiagLds {
+  union {
+    TileLdsT<32> fac;          // the tile factorization
+    float tt[SW][SW + 4];      // the updated tile before factorization
+  } f;
+  _Float16 pan[2][SW][PP];     // [role]: the other factor's tile vectors, [e][k]
+  SliceTab tab;                // the current slice's table
+};
+
+__device__ __forceinline__ unsigned f2u(float v) { return __builtin_bit_cast(unsigned, v); }
+__device__ __forceinline__ float u2f(unsigned v) { return __builtin_bit_cast(float, v); }
+
+// value of v in half HK of the wavefront (lanes 0-31 or 32-63) in every
+// lane: lane n of either half receives lane n of half HK
+template <int HK>
+__device__ __forceinline__ float bcast_half(float v) {
+  const auto r = __builtin_amdgcn_permlane32_swap(f2u(v), f2u(v), false, false);
+  return u2f(HK ? r[1] : r[0]);
+}
+
+__device__ __forceinline__ unsigned pack_h2(float a, float b) {
+  const unsigned lo = __builtin_bit_cast(unsigned short, (_Float16)a);
+  const unsigned hi = __builtin_bit_cast(unsigned short, (_Float16)b);
+  return lo | (hi << 16);
+}
+
+// The pivots of one 32-entry block of a unit, in order (see the header).
+// The tile's rows are read from LDS two pivots ahead of their use; the
+// owner of pivot k finds its scale in its own part of row k (entry k).
+template <int MODE>
+__device__ __forceinline__ void solve_block(float (&x)[16], const float (*m)[32], int lh) {
+  using Sm = Sem<MODE>;
+  constexpr int PF = 3;  // row buffers in flight
+  float4 mv[PF][4];
+  auto fetch = [&](int k) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (8 * q + 7 >= k) mv[k % PF][q] = *(const float4*)&m[k][8 * q + 4 * lh];
+  };
+  fetch(0);
+  fetch(1);
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    // (keeps each row's reads two pivots ahead, not all at the top)
+    asm volatile("" ::: "memory");
+    if (k + 2 < 32) fetch(k + 2);
+    const int hk = (k >> 2) & 1, ik = 4 * (k >> 3) + (k & 3);
+    const float4 d4 = mv[k % PF][k >> 3];
+    const float dk[4] = {d4.x, d4.y, d4.z, d4.w};
+    // (in the owner's half, entry k of row k is the scale)
+    const float own = Sm::mul(x[ik], dk[k & 3]);
+    const float lk = hk ? bcast_half<1>(own) : bcast_half<0>(own);
+    x[ik] = lh == hk ? lk : x[ik];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (8 * q + 7 <= k) continue;  // the group is finished in both halves
+      const float4 v4 = mv[k % PF][q];
+      const float mm[4] = {v4.x, v4.y, v4.z, v4.w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int e0 = 8 * q + j, e1 = e0 + 4;  // entry in half 0 / half 1
+        if (e1 <= k) continue;
+        const float y = Sm::fnma(lk, mm[j], x[4 * q + j]);
+        x[4 * q + j] = (e0 > k || lh) ? y : x[4 * q + j];
+      }
+    }
+  }
+}
+
+// One unit (32 vectors from v0; role 0 = rows of L, 1 = columns of U) for
+// the slice at column so, on one wavefront. T = the role's table and pan
+// the other factor's op-rounded vectors of the tile ([e][k], k < so), both
+// in LDS.
+template <int MODE, int KC>
+__device__ __forceinline__ void do_unit(const BlkArgs& g, const RoleTab& T,
+                                        const _Float16 (*pan)[PP], int role, int64_t v0,
+                                        int so) {
+  using Sm = Sem<MODE>;
+  const int lane = threadIdx.x & 63, l32 = lane & 31, lh = lane >> 5;
+  const int64_t v = v0 + l32;
+  const bool ok = v < g.updm;
+  const int64_t vc = ok ? v : g.updm - 1;
+  const int64_t ldo = g.ldo;
+  // this unit's own earlier entries (the B operand) and its entries of the
+  // buffers, every load in flight at once. The accumulators start at -C, so
+  // that x = -(sum - C) = C - sum (C enters the sum first, as in the update
+  // GEMMs of the drivers).
+  const half_t* own = (role ? g.uop : g.lop) + vc * ldo + 8 * lh;
+  bf16v acc0, acc1;
+  if (role == 0) {
+    // rows of L: entry (v, so + e) of the column-major buffer, a per-lane
+    // offset plus a uniform one per entry (buffer loads, no per-entry
+    // address registers)
+    const auto rc = __builtin_amdgcn_make_buffer_rsrc((void*)(g.pl + (int64_t)so * g.ldpl), 0,
+                                                      (int)(SW * g.ldpl * 4), kBufWord3);
+    const int vo = (int)(vc + 4 * lh * g.ldpl) * 4;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int eu = 8 * (r >> 2) + (r & 3);
+      // (the builtin returns the 32 bits of the entry)
+      acc0[r] = -u2f(__builtin_amdgcn_raw_buffer_load_b32(rc, vo, (int)(eu * g.ldpl * 4), 0));
+      acc1[r] = -u2f(__builtin_amdgcn_raw_buffer_load_b32(rc, vo, (int)((32 + eu) * g.ldpl * 4), 0));
+    }
+  } else {
+    const float* c = g.pu + so + vc * g.ldpu + 4 * lh;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float4 p = *(const float4*)(c + 8 * q);
+      const float4 s = *(const float4*)(c + 32 + 8 * q);
+      acc0[4 * q] = -p.x, acc0[4 * q + 1] = -p.y, acc0[4 * q + 2] = -p.z, acc0[4 * q + 3] = -p.w;
+      acc1[4 * q] = -s.x, acc1[4 * q + 1] = -s.y, acc1[4 * q + 2] = -s.z, acc1[4 * q + 3] = -s.w;
+    }
+  }
+  // the update, KC k-steps of 16 per round trip
+  for (int c0 = 0; c0 < so; c0 += 16 * KC) {
+    bh8 y[KC];
+#pragma unroll
+    for (int u = 0; u < KC; ++u)
+      if (c0 + 16 * u < so) y[u] = *(const bh8*)(own + c0 + 16 * u);
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      if (u % 4 == 0) asm volatile("" ::: "memory");  // (a few LDS reads ahead at a time)
+      if (c0 + 16 * u < so) {
+        const bh8 a0 = *(const bh8*)&pan[l32][c0 + 16 * u + 8 * lh];
+        const bh8 a1 = *(const bh8*)&pan[32 + l32][c0 + 16 * u + 8 * lh];
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(a0, y[u], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(a1, y[u], acc1, 0, 0, 0);
+      }
+    }
+  }
+  float x0[16], x1[16];
+  // block 0
+#pragma unroll
+  for (int r = 0; r < 16; ++r) x0[r] = -acc0[r];
+  solve_block<MODE>(x0, T.m[0], lh);
+  // block 1: its part of the sum over block 0 joins the same accumulator
+  // (B operand: op(x0) rearranged across the halves, see the header)
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks) {
+    const int g0 = 8 * ks, g1 = 8 * ks + 4;  // registers of groups q = 2 ks, 2 ks + 1
+    const unsigned p0 = pack_h2(x0[g0], x0[g0 + 1]), p1 = pack_h2(x0[g0 + 2], x0[g0 + 3]);
+    const unsigned q0 = pack_h2(x0[g1], x0[g1 + 1]), q1 = pack_h2(x0[g1 + 2], x0[g1 + 3]);
+    const auto sa = __builtin_amdgcn_permlane32_swap(p0, q0, false, false);
+    const auto sb = __builtin_amdgcn_permlane32_swap(p1, q1, false, false);
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    const u4 yb = {sa[0], sb[0], sa[1], sb[1]};
+    const bh8 av = *(const bh8*)&T.mot[l32][16 * ks + 8 * lh];
+    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, __builtin_bit_cast(bh8, yb), acc1, 0, 0, 0);
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) x1[r] = -acc1[r];
+  solve_block<MODE>(x1, T.m[1], lh);
+  if (!ok) return;
+  // the factors (fp16) and the op-rounded copy for later slices
+  half_t* oc = (role ? g.uop : g.lop) + vc * ldo + so + 4 * lh;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const bh4 h0 = {(_Float16)x0[4 * q], (_Float16)x0[4 * q + 1], (_Float16)x0[4 * q + 2],
+                    (_Float16)x0[4 * q + 3]};
+    const bh4 h1 = {(_Float16)x1[4 * q], (_Float16)x1[4 * q + 1], (_Float16)x1[4 * q + 2],
+                    (_Float16)x1[4 * q + 3]};
+    *(bh4*)(oc + 8 * q) = h0;
+    *(bh4*)(oc + 32 + 8 * q) = h1;
+    if (role) {
+      half_t* ac = g.a + so + vc * g.lda + 4 * lh;
+      *(bh4*)(ac + 8 * q) = h0;
+      *(bh4*)(ac + 32 + 8 * q) = h1;
+    }
+  }
+  if (!role) {
+    const auto ra = __builtin_amdgcn_make_buffer_rsrc((void*)(g.a + (int64_t)so * g.lda), 0,
+                                                      (int)(SW * g.lda * 2), kBufWord3);
+    const int vo = (int)(vc + 4 * lh * g.lda) * 2;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int eu = 8 * (r >> 2) + (r & 3);
+      __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, (_Float16)x0[r]),
+                                            ra, vo, (int)(eu * g.lda * 2), 0);
+      __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, (_Float16)x1[r]),
+                                            ra, vo, (int)((32 + eu) * g.lda * 2), 0);
+    }
+  }
+}
+
+// the other factor's op-rounded vectors of the slice's tile into LDS:
+// pan[e][k] = src[(so + e) * ldo + k], e < 64, k < so
+__device__ __forceinline__ void stage_panel(_Float16 (*pan)[PP], const half_t* src, int64_t ldo,
+                                            int so) {
+  const int kc = so / 8;  // 16-byte chunks per row
+  for (int c = threadIdx.x; c < SW * kc; c += BLT) {
+    const int e = c / kc, k = 8 * (c % kc);
+    *(bh8*)&pan[e][k] = *(const bh8*)(src + (int64_t)(so + e) * ldo + k);
+  }
+}
+
+template <int MODE>
+__device__ void diagonal_role(const BlkArgs& g, DiagLds& S) {
+  using Sm = Sem<MODE>;
+  const int t = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(t / 64);
+  const int i = t & 63, l32 = i & 31, lh = i >> 5;
+  const int cb = TP * w;
+  const int ns = g.nb / SW;
+  for (int s = 0; s < ns; ++s) {
+    const int so = SW * s;
+    // 0. the slice's operands: op(U) columns of the tile (the rows of L
+    // take them) and op(L) rows of the tile (the columns of U take them),
+    // k < so
+    stage_panel(S.pan[0], g.uop, g.ldo, so);
+    stage_panel(S.pan[1], g.lop, g.ldo, so);
+    __syncthreads();
+    // 1. the tile's update T = C - op(L[so.., 0:so]) op(U[0:so, so..]):
+    // wavefront w takes quarter (ti, tj) = w % 4 over half kh = w / 4 of
+    // the k range; the halves' sums are added, then subtracted from C once
+    {
+      const int qd = w & 3, kh = w >> 2, ti = qd >> 1, tj = qd & 1;
+      const int kl = so / 2;  // k per half (a multiple of 32)
+      bf16v acc;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+      for (int k0 = kh * kl; k0 < kh * kl + kl; k0 += 16) {
+        const bh8 av = *(const bh8*)&S.pan[1][32 * ti + l32][k0 + 8 * lh];
+        const bh8 bv = *(const bh8*)&S.pan[0][32 * tj + l32][k0 + 8 * lh];
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, acc, 0, 0, 0);
+      }
+      const int j = 32 * tj + l32;
+      if (kh == 1) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) S.f.tt[32 * ti + 8 * (r >> 2) + 4 * lh + (r & 3)][j] = acc[r];
+      }
+      __syncthreads();
+      if (kh == 0) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int ii = 32 * ti + 8 * (r >> 2) + 4 * lh + (r & 3);
+          const float c = g.pl[(so + ii) + (int64_t)(so + j) * g.ldpl];
+          S.f.tt[ii][j] = Sm::apply(c, acc[r] + S.f.tt[ii][j]);
+        }
+      }
+    }
+    __syncthreads();
+    float x[TP];
+#pragma unroll
+    for (int jj = 0; jj < TP; ++jj) x[jj] = S.f.tt[i][cb + jj];
+    __syncthreads();
+    // 2. the factorization (ib = 32)
+    tile_factor<MODE, false, 32>(S.f.fac, x, 32, w, i);
+    __syncthreads();
+    const float rk = S.f.fac.R[i];
+    // 3. the factored tile: fp16 factors, the slice table (LDS and global)
+    SliceTab* gt = g.tabs + s;
+#pragma unroll
+    for (int jj = 0; jj < TP; ++jj) g.a[(so + i) + (int64_t)(so + cb + jj) * g.lda] = (half_t)x[jj];
+    auto put = [&](int role, int b, int k, int e, float v) {
+      S.tab.role[role].m[b][k][e] = v;
+      gt->role[role].m[b][k][e] = v;
+    };
+    auto put_o = [&](int role, int e, int k, float v) {
+      const _Float16 o = (_Float16)Sm::op(v);
+      S.tab.role[role].mot[e][k] = o;
+      gt->role[role].mot[e][k] = o;
+    };
+#pragma unroll
+    for (int jj = 0; jj < TP; ++jj) {
+      const int c = cb + jj;
+      // (on the diagonal: the scales, 1 / u_ii for the rows of L, 1 for the
+      // columns of U)
+      const float v0 = c == i ? rk : x[jj], v1 = c == i ? 1.0f : x[jj];
+      if (i < 32 && c < 32) {
+        put(0, 0, i, c, v0);
+        put(1, 0, c, i, v1);
+      } else if (i >= 32 && c >= 32) {
+        put(0, 1, i - 32, c - 32, v0);
+        put(1, 1, c - 32, i - 32, v1);
+      } else if (i < 32) {  // c >= 32: U right of block 0
+        put_o(0, c - 32, i, x[jj]);
+      } else {              // i >= 32, c < 32: L below block 0
+        put_o(1, i - 32, c, x[jj]);
+      }
+    }
+    __syncthreads();
+    // 4. the units inside the diagonal block, rows of L and columns of U
+    // interleaved, the next tile's rows and columns (the first unit pair of
+    // each kind) first
+    const int nun = (g.nb - so - SW) / NV;  // units of each kind (even)
+    for (int q = w; q < 2 * nun; q += BLT / 64) {
+      int role, u;
+      if (q < 4) {
+        role = q >> 1, u = q & 1;
+      } else {
+        role = q & 1, u = 2 + ((q - 4) >> 1);
+      }
+      do_unit<MODE, 14>(g, S.tab.role[role], S.pan[role], role, so + SW + NV * u, so);
+    }
+    __syncthreads();
+  }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(BLT) blockll_diag_kernel(BlkArgs g) {
+  __shared__ DiagLds S;
+  diagonal_role<MODE>(g, S);
+}
+
+// far workgroups 0 .. nfw-1 take rows of L, the others columns of U
+template <int MODE, int KC>
+__device__ __forceinline__ void far_body(const BlkArgs& g) {
+  __shared__ _Float16 pan[SW][PP];
+  __shared__ RoleTab tab;
+  const int t = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(t / 64);
+  const int ns = g.nb / SW;
+  const int role = (int)blockIdx.x >= g.nfw;
+  const int fr = (int)blockIdx.x - role * g.nfw;
+  for (int s = 0; s < ns; ++s) {
+    const int so = SW * s;
+    {
+      const float4* src = (const float4*)&g.tabs[s].role[role];
+      float4* dst = (float4*)&tab;
+      for (int e = t; e < (int)(sizeof(RoleTab) / 16); e += BLT) dst[e] = src[e];
+    }
+    stage_panel(pan, role ? g.lop : g.uop, g.ldo, so);
+    __syncthreads();
+    for (int u = fr * (BLT / 64) + w; u < g.nfu; u += g.nfw * (BLT / 64))
+      do_unit<MODE, KC>(g, tab, pan, role, g.nb + (int64_t)NV * u, so);
+    __syncthreads();
+  }
+}
+// (at most 128 registers: two far workgroups, or one beside an update GEMM
+// workgroup, fit on a CU)
+template <int MODE, int KC>
+__global__ void __launch_bounds__(BLT) __attribute__((amdgpu_waves_per_eu(4)))
+blockll_far_kernel(BlkArgs g) {
+  far_body<MODE, KC>(g);
+}
+
+}  // namespace
+
+// Workspace of the block kernels for an m x m matrix and outer blocks of
+// nb: op(L) rows and op(U) columns (m x nb fp16 each) and the slice tables.
+size_t blockll_workspace_bytes(int64_t m, int64_t nb) {
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  return 2 * al((size_t)m * nb * 2) + al((size_t)(nb / SW) * sizeof(SliceTab));
+}
+
+// whether an outer block of width in (outer blocks of nb) takes the block
+// kernels: ib = 32, TC32 inner updates on fp32 buffers, 64-column slices,
+// 8-byte aligned fp16 columns, buffer offsets within 31 bits
+bool blockll_eligible(int ib, int I, int64_t in, int64_t nb, int64_t m, int64_t lda) {
+  return ib == 32 && I == REMIFA_TC32 && in % SW == 0 && in <= NBM && nb <= NBM &&
+         lda % 4 == 0 && (int64_t)SW * std::max(m, lda) * 4 < (1LL << 31);
+}
+
+// Outer block kk: pl / pu the fp32 buffers, a the fp16 factors at the
+// block's origin, in = block width (<= nb), updm = rows of the block column.
+int blockll_launch(remifa_handle_t h, void* base, int64_t m, int64_t nb, int64_t updm,
+                   int64_t in, const float* pl, int64_t ldpl, const float* pu, int64_t ldpu,
+                   half_t* a, int64_t lda) {
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  char* p = (char*)base;
+  BlkArgs g;
+  g.lop = (half_t*)p;
+  g.uop = (half_t*)(p + al((size_t)m * nb * 2));
+  g.tabs = (SliceTab*)(p + 2 * al((size_t)m * nb * 2));
+  g.ldo = nb;
+  g.updm = updm;
+  g.nb = (int)in;
+  g.pl = pl;
+  g.ldpl = ldpl;
+  g.pu = pu;
+  g.ldpu = ldpu;
+  g.a = a;
+  g.lda = lda;
+  const int64_t far = updm - in;
+  g.nfu = far > 0 ? (int)((far + NV - 1) / NV) : 0;
+  // far workgroups of each role: one unit per wavefront and slice
+  g.nfw = (g.nfu + BLT / 64 - 1) / (BLT / 64);
+  const double bytes = 2.0 * (double)updm * in * (4 + 2 + 2);
+  timer_begin(h, KC_PANEL, bytes, 2);
+  blockll_diag_kernel<SM_F32_TC><<<1, BLT, 0, h->stream>>>(g);
+  RF_HIP(hipGetLastError());
+  timer_end(h);
+  if (g.nfu > 0) {
+    timer_begin(h, KC_PANEL, bytes, 3);
+    blockll_far_kernel<SM_F32_TC, 8><<<2 * g.nfw, BLT, 0, h->stream>>>(g);
+    RF_HIP(hipGetLastError());
+    timer_end(h);
+  }
+  return 0;
+}
+
+}  // namespace remifa

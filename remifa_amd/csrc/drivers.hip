@@ -31,6 +31,11 @@ namespace remifa {
 int subpanel_launch(remifa_handle_t h, int T, int I, bool rl, int ib, int64_t cm, int sw, void* l,
                     int64_t ldl, void* u, int64_t ldu, half_t* cp, int64_t ldcp, float* tab);
 size_t subpanel_table_bytes();
+size_t blockll_workspace_bytes(int64_t m, int64_t nb);
+bool blockll_eligible(int ib, int I, int64_t in, int64_t nb, int64_t m, int64_t lda);
+int blockll_launch(remifa_handle_t h, void* base, int64_t m, int64_t nb, int64_t updm,
+                   int64_t in, const float* pl, int64_t ldpl, const float* pu, int64_t ldpu,
+                   half_t* a, int64_t lda);
 
 namespace {
 
@@ -245,6 +250,8 @@ size_t lu_workspace_bytes(int algo, const remifa_lu_config_t& c, int64_t m, int6
   if (algo == REMIFA_LL && c.W == REMIFA_R_16F && c.IP == REMIFA_R_32F) {
     b += ((size_t)m * nb * 4 + 255) & ~(size_t)255;  // block column buffer
     b += ((size_t)nb * m * 4 + 255) & ~(size_t)255;  // block row buffer
+    if (blockll_eligible((int)c.ib, c.I, nb, nb, m, 4))
+      b += blockll_workspace_bytes(m, nb);  // the block kernels' operands and tables
   }
   if (lookahead_layout(algo, c, n, nb)) {
     // the next block's buffers and the side stream's split-K region
@@ -296,6 +303,12 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
     pl = ar.take((size_t)m * nb * 4);
     pu = ar.take((size_t)nb * m * 4);
   }
+  // the inner factorization of each outer block in two launches
+  // (blockll.hip) where it applies; panel_width set on the handle selects
+  // the slice kernels instead
+  void* blkws = buffered && h->panel_width == 0 && blockll_eligible((int)c.ib, c.I, nb, nb, m, 4)
+                    ? ar.take(blockll_workspace_bytes(m, nb))
+                    : nullptr;
   // lookahead: block kk is factored in buffers plb/pub[kk % 2] while the
   // side stream prepares block kk + 1 in the other pair
   const bool la_layout = lookahead_layout(algo, c, n, nb);
@@ -499,7 +512,11 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
           // inner factorization in fp32; its fp16 mirror is A itself (the
           // reference passes d_a as the fp16 copy, src/lu_nopiv_ll.cxx:632),
           // so the rounded factors land in A as they are produced
-          RF_CHECK(inner_ll(x, REMIFA_R_32F, updm, in, cpl, ldpl, cpu, ldpu, blk, ldda, true));
+          if (blkws && blockll_eligible((int)c.ib, c.I, in, nb, m, ldda))
+            RF_CHECK(blockll_launch(h, blkws, m, nb, updm, in, (const float*)cpl, ldpl,
+                                    (const float*)cpu, ldpu, (half_t*)blk, ldda));
+          else
+            RF_CHECK(inner_ll(x, REMIFA_R_32F, updm, in, cpl, ldpl, cpu, ldpu, blk, ldda, true));
           if (fp8) RF_CHECK(mirror8(ofs, in));
           if (bf16m) {
             // round the finished block column (below the diagonal block) and

@@ -302,6 +302,39 @@ def test_headline_schedule_matches_oracle(handle, gpu, O):
     assert_parity(out[4], out[0], f"headline O={O} split 4 vs auto")
 
 
+@pytest.mark.parametrize("n,nb,ld_pad", [(1000, 256, 4), (1536, 512, 0), (640, 192, 8),
+                                         (777, 128, 3)])
+def test_block_kernel_matches_slices_and_oracle(handle, gpu, n, nb, ld_pad):
+    """The one-launch inner factorization of an outer block (blockll.hip:
+    ib = 32, TC32 inner updates on fp32 buffers, outer blocks of a multiple
+    of 64 columns, fp16 columns 8-byte aligned) against the oracle, and
+    against the slice kernels it replaces (panel_width = 64 selects them).
+    Cases: a ragged last block (1000 = 3 x 256 + 232 and 777 = 6 x 128 + 9
+    take the slice kernels there), units cut by the matrix edge, nb = 192
+    (three slices), and a leading dimension that is not a multiple of 4
+    (every block on the slice kernels)."""
+    import torch
+    rf = _rf()
+    a32 = orc.gen_diagdom_alpha(n, 7 + n, 1.1)
+    a16 = orc.f32_to_f16_bits(a32)
+    cfg = rf.api.make_config(W=rf.R_16F, I=rf.TC32, O=rf.TC32, OP=rf.R_32F, ib=32)
+    out = {}
+    try:
+        for pw in (0, 64):
+            handle.set_tuning(pw, 0)
+            da = to_dev(a16, gpu, ld=n + ld_pad)
+            assert rf.lu_nopiv(handle, da, nb, algo=rf.LL, cfg=cfg) == 0
+            torch.cuda.synchronize()
+            out[pw] = to_np(da)
+    finally:
+        handle.set_tuning(0, 0)
+    o = orc.lu_nopiv("ll", orc.cfg(W=orc.F16, I=orc.TC32, O=orc.TC32, OP=orc.F32, ib=32), a16, nb)
+    f64 = orc.lu_f64(orc.round_to(orc.F16, a32))
+    _check(out[0], o, f64, orc.F16, f"block kernel n={n} nb={nb}")
+    _check(out[64], o, f64, orc.F16, f"slice kernels n={n} nb={nb}")
+    assert_parity(out[0], out[64], f"block kernel vs slice kernels n={n} nb={nb}")
+
+
 @pytest.mark.parametrize("alpha", [1.01, 1.1, 2.0])
 def test_c2_fp32_storage_fp16_update_alpha_sweep(handle, gpu, alpha):
     """Config 2 at n = 2048: fp32 storage with TC32 outer updates
