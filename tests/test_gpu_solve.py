@@ -2,7 +2,8 @@
 
 getrs: the GPU solve with fp32 (or fp16) factors must agree with the FP64
 substitution on the same factors within a backward-stable bound
-|x - x64| <= 64 n u |x64|_inf scale. The FP64 residual is checked against
+|x - x64| <= 64 n u |x64|_inf scale (up to 32 dependent 128-row block rows
+at n = 4096). The FP64 residual is checked against
 the oracle's row-ordered FP64 sum within n * u_64 |A||x|. IR must reach the
 normwise backward error 1e-14 (tests/errors.hxx measure, recomputed by the
 oracle on the host).
@@ -22,7 +23,8 @@ def _rf():
 
 
 @pytest.mark.parametrize("n,nrhs,lu_t", [(2, 1, "f32"), (63, 1, "f32"), (64, 3, "f32"),
-                                         (300, 16, "f32"), (513, 20, "f16"), (1000, 1, "f16")])
+                                         (300, 16, "f32"), (513, 20, "f16"), (1000, 1, "f16"),
+                                         (2500, 1, "f32"), (3000, 1, "f16"), (4096, 1, "f16")])
 @pytest.mark.parametrize("trans", [False, True])
 def test_getrs_matches_f64(handle, gpu, n, nrhs, lu_t, trans):
     import torch
@@ -63,7 +65,8 @@ def test_residual_f64(handle, gpu):
     assert np.all(np.abs(r - ref) <= 2 * bound)
 
 
-@pytest.mark.parametrize("n,nrhs,alpha", [(512, 1, 1.05), (512, 16, 1.05), (1000, 4, 1.2)])
+@pytest.mark.parametrize("n,nrhs,alpha", [(512, 1, 1.05), (512, 16, 1.05), (1000, 4, 1.2),
+                                           (2048, 1, 1.05)])
 def test_ir_reaches_1e14(handle, gpu, n, nrhs, alpha):
     import torch
     rf = _rf()
@@ -81,4 +84,8 @@ def test_ir_reaches_1e14(handle, gpu, n, nrhs, alpha):
     x = to_np(dx)
     host_err = orc.backward_error(a, b, x)
     assert host_err <= 1e-14
-    assert abs(host_err - be) <= 1e-15 + 0.1 * host_err
+    # the GPU's own measure agrees with the host's; at convergence both
+    # residuals are mostly the rounding of their own n-term FP64 sums
+    # (different orders), so they agree to a few units of 2^-53 in the
+    # normwise measure, not to a fraction of it
+    assert abs(host_err - be) <= 4e-15 + 0.1 * host_err
