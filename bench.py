@@ -332,8 +332,17 @@ def bench_batched(a):
     flops = float((2.0 * sizes.astype(np.float64) ** 3 / 3.0).sum())
     bytes_ = float((8.0 * sizes.astype(np.float64) ** 2).sum())
 
+    # size buckets (the batch is sorted largest first): one persistent
+    # launch per bucket, its kernel shape from the autotuned table
+    buckets = []
+    for lo_b, bm in ((256, 512), (128, 256), (64, 128), (0, 64)):
+        idx = np.nonzero((sizes > lo_b) & (sizes <= bm))[0]
+        if len(idx):
+            buckets.append((int(idx[0]), int(idx[-1]) + 1, bm))
+
     def run():
-        rf.lu_nopiv_batched(h, rf.TC32, n_t, ld_t, ptrs, info, 512)
+        for i0, i1, bm in buckets:
+            rf.lu_nopiv_batched(h, rf.TC32, n_t[i0:i1], ld_t[i0:i1], ptrs[i0:i1], info[i0:i1], bm)
         rf.getrs_nopiv_batched(h, n_t, ld_t, ptrs, bptrs, 512)
 
     work.copy_(a0); bw.copy_(b0)
@@ -397,8 +406,10 @@ def bench_batched(a):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32 storage, fp16 MFMA updates (fp32 accumulate)",
         "data": "synthetic: seeded minstd_rand0 uniform(-1,1), a_ii = 1.2*sum|a_ij|",
-        "config": {"workload": f"{a.count} LUs + solves, one persistent launch each, "
-                               "HIP graph", "count": a.count, "seed": a.seed, "panel": 32},
+        "config": {"workload": f"{a.count} LUs + solves: one persistent LU launch per size "
+                               "bucket (autotuned shape) and one solve launch, in one HIP graph",
+                   "count": a.count, "seed": a.seed, "panel": 32,
+                   "buckets": [[bm, i1 - i0] for i0, i1, bm in buckets]},
         "flops": flops, "matrices_per_s": round(a.count / (t * 1e-3), 1),
         "roofline": {"bound": "hbm", "algorithmic_bytes": bytes_,
                      "work": "sum 8 n_i^2 bytes (fp32 read + write once)",

@@ -354,11 +354,19 @@ int remifa_conv_error(remifa_handle_t h, int conv_t, int64_t m, int64_t n, const
                       int64_t lda, const void *d_conv, int64_t ldc, double *out);
 
 /* ------------------------------------------------------------ batched */
-/* Factor `count` independent matrices of orders n[i] (<= 512) stored at
- * d_ptrs[i] (fp32, leading dimension ld[i]) in place, in one persistent
- * launch. d_n, d_ld, d_ptrs are device arrays. No reference counterpart. */
+/* Factor `count` independent matrices of orders n[i] (<= max_n <= 512)
+ * stored at d_ptrs[i] (fp32, leading dimension ld[i]) in place, in one
+ * persistent launch. d_n, d_ld, d_ptrs are device arrays. max_n picks the
+ * size bucket (64, 128, 256, 512) whose kernel runs; a caller with mixed
+ * sizes gets the best schedule by sorting the batch by order and calling
+ * once per bucket. No reference counterpart (NEW row of the hot path). */
 int remifa_lu_nopiv_batched(remifa_handle_t h, int ctype, int64_t count, const int *d_n,
                             const int *d_ld, float *const *d_ptrs, int *d_info, int max_n);
+/* Threads and persistent workgroups per CU of remifa_lu_nopiv_batched per
+ * size bucket: rules[3i..3i+2] = (bucket 64 | 128 | 256 | 512, threads
+ * (64 or 128 | 128 or 256 | 256 or 512 | 512), workgroups per CU 1..16);
+ * count = 0: defaults. Filled by the autotuner (remifa_amd/autotune.py). */
+int remifa_set_batched_tuning(remifa_handle_t h, int count, const int *rules);
 /* Solve A_i x_i = b_i for every matrix of the batch with its factors
  * (fp32 right-hand sides, one per matrix, overwritten). */
 int remifa_getrs_nopiv_batched(remifa_handle_t h, int64_t count, const int *d_n, const int *d_ld,

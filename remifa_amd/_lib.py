@@ -96,6 +96,7 @@ SIGNATURES = {
     "remifa_conv_error": [_vp, _i, _i64, _i64, _vp, _i64, _vp, _i64, _pd],
     "remifa_lu_nopiv_batched": [_vp, _i, _i64, _vp, _vp, _vp, _vp, _i],
     "remifa_getrs_nopiv_batched": [_vp, _i64, _vp, _vp, _vp, _vp, _i],
+    "remifa_set_batched_tuning": [_vp, _i, _vp],
 }
 
 

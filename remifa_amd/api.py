@@ -66,10 +66,20 @@ class Handle:
         self.stream = s
         check(lib().remifa_create(C.byref(self._h), C.c_void_p(s.cuda_stream)), "remifa_create")
         if load_tuning:
-            from .autotune import load_rules
+            from .autotune import load_batched_rules, load_rules
             rules = load_rules()
             if rules:
                 self.set_gemm_rules(rules)
+            brules = load_batched_rules()
+            if brules:
+                self.set_batched_tuning(brules)
+
+    def set_batched_tuning(self, rules):
+        """batched factorization shape per size bucket: list of [bucket,
+        threads, workgroups per CU] (remifa_set_batched_tuning)"""
+        flat = (C.c_int * (3 * len(rules)))(*[int(v) for r in rules for v in r])
+        check(lib().remifa_set_batched_tuning(self._h, len(rules), C.cast(flat, C.c_void_p)),
+              "remifa_set_batched_tuning")
 
     def set_gemm_rules(self, rules):
         """autotuned GEMM table: list of [kind, m_lo, m_hi, n_lo, n_hi, tile, split]"""

@@ -13,7 +13,11 @@ keeps the fastest, and writes a rule table (JSON). A Handle loads the table
 at creation (remifa_set_gemm_rules) unless REMIFA_TUNING=off; the table
 committed next to this file was measured on an MI355X.
 
+It also picks, per size bucket of the batched factorization, the thread
+count and persistent workgroups per CU (remifa_set_batched_tuning).
+
 usage (on a GPU): python -m remifa_amd.autotune [--n 32768] [--nb 512] [--out FILE]
+                  [--batched-only]
 """
 from __future__ import annotations
 
@@ -131,14 +135,83 @@ def load_rules(path: str = TABLE):
     return rules
 
 
+# batched factorization (remifa_lu_nopiv_batched): per size bucket, the
+# thread counts its kernel is built for and the persistent workgroups per CU
+BATCHED = {64: ([64, 128], [2, 4, 8]), 128: ([128, 256], [1, 2, 4]),
+           256: ([256, 512], [1, 2, 3]), 512: ([512], [1, 2])}
+
+
+def tune_batched(count: int = 4000, verbose: bool = True) -> list:
+    """[[bucket, threads, per_cu], ...]: for every size bucket, the fastest
+    candidate on `count` matrices with orders uniform over the bucket
+    (alpha = 1.2, fp32 storage, TC32 updates)"""
+    import numpy as np
+    import torch
+
+    import remifa_amd as rf
+
+    dev = torch.device("cuda:0")
+    h = rf.Handle(load_tuning=False)
+    rng = np.random.default_rng(7)
+    rules, lo = [], 16
+    for bm, (threads, per_cus) in BATCHED.items():
+        sizes = np.sort(rng.integers(lo, bm + 1, count))[::-1].copy()
+        lo = bm + 1
+        offs = np.concatenate([[0], np.cumsum(sizes.astype(np.int64) ** 2)[:-1]])
+        a0 = torch.empty(int((sizes.astype(np.int64) ** 2).sum()), dtype=torch.float32, device=dev)
+        for i, (n, off) in enumerate(zip(sizes, offs)):
+            rf.gen_diagdom(h, a0[off:off + n * n].view(int(n), int(n)).t(), seed=i, alpha=1.2)
+        work = torch.empty_like(a0)
+        ptrs = torch.tensor(work.data_ptr() + 4 * offs, dtype=torch.int64, device=dev)
+        n_t = torch.tensor(sizes, dtype=torch.int32, device=dev)
+        info = torch.zeros(count, dtype=torch.int32, device=dev)
+        best = None
+        for bt in threads:
+            for pc in per_cus:
+                h.set_batched_tuning([[bm, bt, pc]])
+                ms = []
+                for _ in range(3):
+                    work.copy_(a0)
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record(h.stream)
+                    rf.lu_nopiv_batched(h, rf.TC32, n_t, n_t, ptrs, info, bm)
+                    e1.record(h.stream)
+                    torch.cuda.synchronize()
+                    ms.append(e0.elapsed_time(e1))
+                t = min(ms)
+                if verbose:
+                    print(f"batched bucket {bm:3d} threads {bt:3d} per_cu {pc}: {t:.3f} ms", flush=True)
+                if best is None or t < best[0]:
+                    best = (t, bt, pc)
+        rules.append([bm, best[1], best[2]])
+    return rules
+
+
+def load_batched_rules(path: str = TABLE):
+    """batched rules of a table file, or None"""
+    if os.environ.get("REMIFA_TUNING", "").lower() == "off" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("batched")
+
+
 def main():
     p = argparse.ArgumentParser()
     p.add_argument("--n", type=int, default=32768)
     p.add_argument("--nb", type=int, default=512)
     p.add_argument("--out", default=TABLE)
+    p.add_argument("--batched-only", action="store_true",
+                   help="retune only the batched factorization (keep the GEMM rules)")
     a = p.parse_args()
-    t = tune(a.n, a.nb, a.out)
-    print(json.dumps(t["best"]))
+    if a.batched_only and os.path.exists(a.out):
+        with open(a.out) as f:
+            t = json.load(f)
+    else:
+        t = tune(a.n, a.nb, a.out)
+    t["batched"] = tune_batched()
+    with open(a.out, "w") as f:
+        json.dump(t, f, indent=1)
+    print(json.dumps({"best": t["best"], "batched": t["batched"]}))
 
 
 if __name__ == "__main__":

@@ -84,6 +84,8 @@ struct remifa_handle_st {
   // autotuned GEMM rules, first match wins: {opk (-1 any), m_lo, m_hi, n_lo,
   // n_hi, tile (-1 heuristic), split (0 heuristic)}, bounds inclusive
   std::vector<std::array<int64_t, 7>> gemm_rules;
+  // batched factorization shape per size bucket: {bound, threads, per CU}
+  std::vector<std::array<int, 3>> batched_rules;
   // device address of this device's static scratch array of the
   // no-allocation entry point (solve.hip), looked up at creation
   void* scratch = nullptr;
