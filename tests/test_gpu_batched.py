@@ -95,6 +95,41 @@ def test_batched_result_independent_of_batch(handle, gpu):
         assert np.array_equal(got1[i].view(np.uint32), got2[j].view(np.uint32))
 
 
+@pytest.mark.parametrize("bucket,threads,per_cu", [(64, 64, 4), (64, 128, 2), (128, 128, 4),
+                                                   (128, 256, 1), (256, 256, 2), (256, 512, 1),
+                                                   (512, 512, 2)])
+def test_batched_every_bucket_kernel_matches_oracle(handle, gpu, bucket, threads, per_cu):
+    """every kernel the tuning table can pick (size bucket x threads x
+    workgroups per CU): ragged orders inside the bucket, against the oracle;
+    the factors do not depend on the shape (same operations per entry)"""
+    rng = np.random.default_rng(bucket + threads)
+    lo = {64: 2, 128: 65, 256: 129, 512: 257}[bucket]
+    sizes = list(rng.integers(lo, bucket + 1, 5)) + [bucket]
+    mats = _batch(sizes, 300 + bucket)
+    try:
+        handle.set_batched_tuning([[bucket, threads, per_cu]])
+        got, info = _run(handle, gpu, mats)
+        handle.set_batched_tuning([[bucket, bucket, 1]])
+        ref, _ = _run(handle, gpu, mats)
+    finally:
+        handle.set_batched_tuning([])
+    assert np.all(info == 0)
+    c = orc.cfg(W=orc.F32, I=orc.TC32, O=orc.TC32, ib=32)
+    for a, lu, r in zip(mats, got, ref):
+        assert_parity(lu, orc.lu_nopiv("rl", c, a, 32), f"bucket {bucket} n={a.shape[0]}")
+        assert np.array_equal(lu.view(np.uint32), r.view(np.uint32))
+
+
+def test_batched_tuning_rejects_bad_rules(handle, gpu):
+    import remifa_amd as rf
+    for bad in ([[96, 128, 1]], [[128, 64, 1]], [[512, 256, 1]], [[256, 256, 0]],
+                [[64, 64, 17]]):
+        with pytest.raises(rf.RemifaError):
+            handle.set_batched_tuning(bad)
+    handle.set_batched_tuning([[512, 512, 2], [64, 128, 8]])
+    handle.set_batched_tuning([])
+
+
 def test_batched_solve(handle, gpu):
     import torch
     import remifa_amd as rf
