@@ -662,25 +662,31 @@ def bench_headline(a):
     # this workload (FETCH_SIZE + WRITE_SIZE per dispatch, calibrated)
     traffic, traffic_src = None, None
     bm, bn = rf.api._GEMM_TILE.get(dom & 15, (0, 0, 0, 0))[:2]
-    prof = os.path.join(ROOT, "profiles", f"gemm_mfma_{bm}x{bn}_lu_n{n}_pmc.md")
+    kname = "gemm_big" if (dom & 15) == 4 else "gemm_mfma"
+    prof = os.path.join(ROOT, "profiles", f"{kname}_{bm}x{bn}_lu_n{n}_pmc.md")
     if os.path.exists(prof):
         import re
         m = re.search(r"traffic_bytes_per_launch = (\d+)", open(prof).read())
         if m:
             traffic, traffic_src = int(m.group(1)), os.path.relpath(prof, ROOT)
 
-    # algorithmic bytes of the wide updates (the 128x128 launches of LL): A
-    # and B in fp16 once, the fp32 panel buffer C read and written once
+    # algorithmic bytes of the dominant kernel's launches among the wide
+    # updates of LL: A and B in fp16 once, the fp32 panel buffer C read and
+    # written once. The 256x256 kernel takes the launches with K >= 2048, the
+    # 128x128 one the shorter ones (every one for the FP8 kernel).
     algo_bytes = None
     if algo == rf.LL:
         tot, cnt = 0, 0
         # (with the lookahead block k's update is two launches of each shape:
         # K = ofs - nb on the second stream, then K = nb for block k-1)
         la = not a.no_lookahead and (a.update != "fp8" or nb % 128 == 0)
+        dom_big, dom_fp8 = (dom & 15) == 4, (dom >> 8) == 3
         for k in range(1, (n + nb - 1) // nb):
             ofs = k * nb
             ks = ([ofs - nb] if k >= 2 else []) + [nb] if la else [ofs]
             for K in ks:
+                if not dom_fp8 and (K >= 2048) != dom_big:
+                    continue
                 for (M, N) in ((n - ofs, min(nb, n - ofs)), (min(nb, n - ofs), n - ofs)):
                     tot += 2 * (M * K + K * N) + 8 * M * N
                     cnt += 1

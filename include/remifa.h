@@ -92,8 +92,10 @@ int remifa_lu_workspace_size(remifa_handle_t handle, int algo, const remifa_lu_c
 int remifa_set_tuning(remifa_handle_t handle, int panel_width, int split_k);
 /* Force the output tile of the update GEMMs (-1 = chosen per shape):
  * 0: 256x32, 1: 32x256, 2: 256x64, 3: 64x256, 4: 256x256, 5: 128x128,
- * 6: 256x128, 7: 128x256 (fp32 operands only take 0, 1, 5). Used by the
- * autotuner. */
+ * 6: 256x128, 7: 128x256 (fp32 operands only take 0, 1, 5). Tile 4 runs
+ * the LDS-DMA kernel for fp16 operands or bf16 storage when M and N are
+ * multiples of 256, K of 32 and the operands 16-byte aligned (else 5); a
+ * rule's tile 4 also falls back to 5 below K = 2048. Used by the autotuner. */
 int remifa_set_gemm_tile(remifa_handle_t handle, int tile);
 /* Autotuned GEMM table: count rules of 7 int64 each, {operand kind (0 fp16,
  * 1 bf16, 2 fp32; -1 any), m_lo, m_hi, n_lo, n_hi, tile (-1 = heuristic),

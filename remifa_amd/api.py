@@ -154,7 +154,7 @@ class Handle:
 # GEMM variant codes of the timer: opk << 8 | fp16-output << 4 | tile
 _GEMM_OPK = {0: "fp16", 1: "bf16", 2: "fp32", 4: "bf16-stored"}
 _GEMM_TILE = {0: (256, 32, 4, 1), 1: (32, 256, 1, 4), 2: (256, 64, 4, 1), 3: (64, 256, 1, 4),
-              4: (256, 256, 2, 2), 5: (128, 128, 2, 2), 6: (256, 128, 2, 2), 7: (128, 256, 2, 2)}
+              4: (256, 256, 2, 4), 5: (128, 128, 2, 2), 6: (256, 128, 2, 2), 7: (128, 256, 2, 2)}
 
 
 def gemm_variant_name(v: int) -> str:
@@ -169,6 +169,9 @@ def gemm_variant_name(v: int) -> str:
     if tile == 8:
         return (f"gemm_pair_kernel<{opk}, {'_Float16' if out16 else 'float'}> "
                 f"({_GEMM_OPK.get(opk, '?')} operands, 256x32 + 32x256 tiles)")
+    if tile == 4:
+        return (f"gemm_big_kernel<{opk}, {'_Float16' if out16 else 'float'}, 32, 4, 8> "
+                f"({_GEMM_OPK.get(opk, '?')} operands, 256x256 tile, LDS-DMA slabs)")
     return (f"gemm_mfma_kernel<{opk}, {'_Float16' if out16 else 'float'}, {bm}, {bn}, {wm}, {wn}>"
             f" ({_GEMM_OPK.get(opk, '?')} operands, {bm}x{bn} tile)")
 

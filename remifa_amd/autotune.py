@@ -29,11 +29,11 @@ import time
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_gfx950.json")
 
 # tile ids (remifa_set_gemm_tile): 0 256x32, 1 32x256, 2 256x64, 3 64x256,
-# 5 128x128, 6 256x128, 7 128x256
+# 4 256x256 (LDS-DMA kernel), 5 128x128, 6 256x128, 7 128x256
 FAMILIES = {
     "narrow_n": {"tiles": [0, 2, 5], "splits": [0, 1, 2, 4, 8]},
     "narrow_m": {"tiles": [1, 3, 5], "splits": [0, 1, 2, 4, 8]},
-    "wide": {"tiles": [5, 6, 7], "splits": [0, 1, 2]},
+    "wide": {"tiles": [4, 5, 6, 7], "splits": [0, 1, 2]},
 }
 
 

@@ -35,6 +35,7 @@ namespace remifa {
 namespace {
 
 #include "gemm_mfma.hpp"
+#include "gemm_big.hpp"
 #include "gemm_fp8.hpp"
 
 // C = beta*C + sum_z W[z] (slices added in order z = 0, 1, ...)
@@ -209,6 +210,30 @@ int split_model(int64_t M, int64_t N, int64_t K, TileGeom tg, int num_cu, double
   return best;
 }
 
+// 256 x 256 tiles on gemm_big_kernel: 16-bit operands in memory, M and N
+// multiples of 256, K a multiple of the 32-deep slab (split-K chunks are
+// multiples of 64), 16-byte aligned operands and column offsets of one slab
+// within 32 bits (the DMA lane offsets)
+bool big_fits(int opk, int64_t m, int64_t n, int64_t k, const void* a, int64_t lda, const void* b,
+              int64_t ldb) {
+  return (opk == OPK_F16 || opk == OPK_BF16S) && m % 256 == 0 && n % 256 == 0 && k % 32 == 0 &&
+         (uintptr_t)a % 16 == 0 && (uintptr_t)b % 16 == 0 && lda % 8 == 0 && ldb % 8 == 0 &&
+         (64 * lda + 256) * 2 < (int64_t)1 << 31 && (256 * ldb + 64) * 2 < (int64_t)1 << 31;
+}
+
+// 32-deep slabs, four LDS stages (128 KB), eight wavefronts: with four the
+// LDS-DMA requests stalled the one wavefront per SIMD that also issues the
+// matrix-core work (964 -> 1265 TFLOP/s at 8192^3 with eight, MI355X)
+template <int OPK, typename OutT>
+int launch_big(remifa_handle_t h, GemmArgs g, int split) {
+  g.tiles_m = (int)(g.M / 256);
+  g.tiles_n = (int)(g.N / 256);
+  dim3 grid((unsigned)(g.tiles_m * g.tiles_n), 1, (unsigned)split);
+  gemm_big_kernel<OPK, OutT, 32, 4, 8><<<grid, 512, 0, h->stream>>>(g);
+  RF_HIP(hipGetLastError());
+  return 0;
+}
+
 template <int OPK, typename OutT>
 int dispatch_basic(remifa_handle_t h, int tile, GemmArgs g, int split, bool vec) {
   switch (tile) {
@@ -229,7 +254,6 @@ int dispatch_tiles(remifa_handle_t h, int tile, GemmArgs g, int split, bool vec)
     switch (tile) {
       case T256x64: return launch_mfma<OPK, OutT, 256, 64, 4, 1>(h, g, split, vec);
       case T64x256: return launch_mfma<OPK, OutT, 64, 256, 1, 4>(h, g, split, vec);
-      case T256x256: return launch_mfma<OPK, OutT, 256, 256, 2, 2>(h, g, split, vec);
       case T256x128: return launch_mfma<OPK, OutT, 256, 128, 2, 2>(h, g, split, vec);
       case T128x256: return launch_mfma<OPK, OutT, 128, 256, 2, 2>(h, g, split, vec);
     }
@@ -539,7 +563,11 @@ int gemm_launch_cin(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t m
     return t >= 0 && (opk != OPK_F32 || t == T256x32 || t == T32x256 || t == T128x128);
   };
   if (fits(rule_tile)) tile = rule_tile;  // autotuned table
+  // below K = 2048 the 256 x 256 kernel's start-up and epilogue outweigh its
+  // loop (measured on MI355X at N = 512): 128 x 128 unless forced
+  if (tile == T256x256 && g.K < 2048) tile = T128x128;
   if (fits(h->gemm_tile)) tile = h->gemm_tile;  // explicit override (autotuner runs)
+  if (tile == T256x256 && !big_fits(opk, m, n, g.K, a, lda, b, ldb)) tile = T128x128;
 
   // split-K: the caller's, the rule's, else the cost model's
   int split = h->split_k > 0 ? h->split_k : rule_split;
@@ -573,7 +601,14 @@ int gemm_launch_cin(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t m
 
   timer_begin(h, KC_GEMM, flops, (opk << 8) | ((out_t == REMIFA_R_16F) << 4) | tile);
   int st;
-  if (opk == OPK_F32) {
+  if (tile == T256x256) {
+    if (opk == OPK_F16)
+      st = out_t == REMIFA_R_32F ? launch_big<OPK_F16, float>(h, g, split)
+                                 : launch_big<OPK_F16, half_t>(h, g, split);
+    else
+      st = out_t == REMIFA_R_32F ? launch_big<OPK_BF16S, float>(h, g, split)
+                                 : launch_big<OPK_BF16S, half_t>(h, g, split);
+  } else if (opk == OPK_F32) {
     st = out_t == REMIFA_R_32F ? dispatch_tiles<OPK_F32, float>(h, tile, g, split, vec)
                                : dispatch_tiles<OPK_F32, half_t>(h, tile, g, split, vec);
   } else if (opk == OPK_F16) {

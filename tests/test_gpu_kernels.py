@@ -225,6 +225,54 @@ def test_gemm_every_tile_within_bound(handle, gpu, tile, ctype, m, n, k):
     assert np.all(np.abs(got.astype(np.float64) - ref) <= bound + 1e-30)
 
 
+@pytest.mark.parametrize("m,n,k,out,split,storage", [
+    (512, 512, 256, "f32", 0, "f16"),
+    (768, 512, 1536, "f32", 0, "f16"),    # 48 slabs: the steady-state loop
+    (256, 1024, 64, "f32", 0, "f16"),     # fewer slabs than the prefetch depth
+    (256, 256, 32, "f32", 0, "f16"),      # a single slab
+    (512, 256, 128, "f16", 0, "f16"),     # fp16 output
+    (1024, 512, 2048, "f32", 4, "f16"),   # split-K partials
+    (512, 768, 512, "f32", 0, "bf16"),    # bf16 storage
+])
+def test_gemm_large_tile_within_bound(handle, gpu, m, n, k, out, split, storage):
+    """the 256 x 256 tile kernel (LDS-DMA operand slabs, swizzled layouts)
+    on shapes it accepts, with padded leading dimensions"""
+    import torch
+    rf = _rf()
+    rng = np.random.default_rng(m + 3 * n + k)
+    a, b, c = _gemm_inputs(rng, m, n, k, True)
+    if storage == "bf16":
+        a, b = orc.round_to(orc.BF16, a), orc.round_to(orc.BF16, b)
+        bits = lambda x: (np.asfortranarray(x).view(np.uint32) >> 16).astype(np.uint16)
+        da = to_dev(bits(a), gpu, ld=m + 8).view(torch.bfloat16)
+        db = to_dev(bits(b), gpu, ld=k + 16).view(torch.bfloat16)
+        ct, oc, opp = rf.BF16_TC32, orc.BF16TC32, orc.BF16
+    else:
+        da = to_dev(orc.f32_to_f16_bits(a), gpu, ld=m + 8)
+        db = to_dev(orc.f32_to_f16_bits(b), gpu, ld=k + 16)
+        ct, oc, opp = rf.TC32, orc.TC32, orc.F16
+    if out == "f16":
+        c = orc.round_to(orc.F16, c)
+    dc = to_dev(orc.f32_to_f16_bits(c) if out == "f16" else c, gpu, ld=m + 4)
+    handle.set_gemm_tile(4)
+    handle.set_tuning(0, split)
+    try:
+        rf.gemmex(handle, ct, da, db, dc, alpha=-1.0, beta=1.0)
+        torch.cuda.synchronize()
+    finally:
+        handle.set_gemm_tile(-1)
+        handle.set_tuning(0, 0)
+    got = to_np(dc)
+    got = orc.f16_bits_to_f32(got) if out == "f16" else got
+    ref = orc.gemm_sub(oc, a, b, c, orc.F16 if out == "f16" else orc.F32)
+    aa, bb = orc.round_to(opp, a).astype(np.float64), orc.round_to(opp, b).astype(np.float64)
+    gam = (k + 1) * 2.0**-24 / (1 - (k + 1) * 2.0**-24)
+    bound = 2 * gam * (np.abs(aa) @ np.abs(bb) + np.abs(c))
+    if out == "f16":
+        bound += 2 * 2.0**-11 * np.abs(ref)
+    assert np.all(np.abs(got.astype(np.float64) - ref) <= bound + 1e-30)
+
+
 @pytest.mark.parametrize("ctype", ["FP16", "TC16"])
 @pytest.mark.parametrize("m,n,k", [(1, 1, 1), (37, 45, 19), (130, 70, 200), (64, 64, 257)])
 def test_gemm_h16_bitexact(handle, gpu, ctype, m, n, k):

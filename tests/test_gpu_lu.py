@@ -335,6 +335,38 @@ def test_block_kernel_matches_slices_and_oracle(handle, gpu, n, nb, ld_pad):
     assert_parity(out[0], out[64], f"block kernel vs slice kernels n={n} nb={nb}")
 
 
+@pytest.mark.parametrize("n,nb,ld_pad,lookahead", [(1536, 512, 0, True), (2048, 256, 8, True),
+                                                   (1536, 512, 0, False), (1280, 256, 0, True)])
+def test_large_tile_updates_match_oracle(handle, gpu, n, nb, ld_pad, lookahead):
+    """The headline layout (fp16 storage, fp32 buffers, TC32) with every
+    outer update forced onto the 256 x 256 LDS-DMA kernel (the default
+    takes it only from K = 2048, which these orders do not reach): the
+    second stream's K = ofs - nb part, the K = nb part, split-K by the cost
+    model, and the unsplit path without the lookahead. Against the oracle
+    and against the default tiles."""
+    import torch
+    rf = _rf()
+    a32 = orc.gen_diagdom_alpha(n, 11 + n, 1.1)
+    a16 = orc.f32_to_f16_bits(a32)
+    cfg = rf.api.make_config(W=rf.R_16F, I=rf.TC32, O=rf.TC32, OP=rf.R_32F, ib=32)
+    out = {}
+    try:
+        handle.set_lookahead(lookahead, 0)
+        for tile in (4, -1):
+            handle.set_gemm_tile(tile)
+            da = to_dev(a16, gpu, ld=n + ld_pad)
+            assert rf.lu_nopiv(handle, da, nb, algo=rf.LL, cfg=cfg) == 0
+            torch.cuda.synchronize()
+            out[tile] = to_np(da)
+    finally:
+        handle.set_gemm_tile(-1)
+        handle.set_lookahead(True, 0)
+    o = orc.lu_nopiv("ll", orc.cfg(W=orc.F16, I=orc.TC32, O=orc.TC32, OP=orc.F32, ib=32), a16, nb)
+    f64 = orc.lu_f64(orc.round_to(orc.F16, a32))
+    _check(out[4], o, f64, orc.F16, f"256x256 updates n={n} nb={nb}")
+    assert_parity(out[4], out[-1], f"256x256 vs default tiles n={n} nb={nb}")
+
+
 @pytest.mark.parametrize("alpha", [1.01, 1.1, 2.0])
 def test_c2_fp32_storage_fp16_update_alpha_sweep(handle, gpu, alpha):
     """Config 2 at n = 2048: fp32 storage with TC32 outer updates
