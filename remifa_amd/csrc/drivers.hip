@@ -200,6 +200,17 @@ bool lookahead_layout(int algo, const remifa_lu_config_t& c, int64_t n, int64_t 
   return algo == REMIFA_LL && c.W == REMIFA_R_16F && c.IP == REMIFA_R_32F &&
          c.OP == REMIFA_R_32F && (c.O != REMIFA_FP8_TC32 || nb % 128 == 0) && n > nb;
 }
+// LL lookahead on the factors in place, fp32 storage with fp32-accumulating
+// outer updates: the update of block k+1 by blocks 0..k-1 writes only block
+// k+1's column and row of A (and reads factors left of / above block k),
+// block k's factorization only block k's column and row. The update then
+// reaches A in two parts, one fp32 rounding more; with fp16 storage or fp16
+// accumulation that extra rounding would be an fp16 one, so those keep the
+// single update.
+bool lookahead_inplace(int algo, const remifa_lu_config_t& c, int64_t n, int64_t nb) {
+  return algo == REMIFA_LL && c.W == REMIFA_R_32F &&
+         (c.O == REMIFA_FP32 || c.O == REMIFA_TC32 || c.O == REMIFA_BF16_TC32) && n > nb;
+}
 
 // The lookahead's second stream (default priority), masked to leave
 // h->side_reserve CUs to the handle's stream. The mask bits i = x + 8 y
@@ -253,6 +264,7 @@ size_t lu_workspace_bytes(int algo, const remifa_lu_config_t& c, int64_t m, int6
     if (blockll_eligible((int)c.ib, c.I, nb, nb, m, 4))
       b += blockll_workspace_bytes(m, nb);  // the block kernels' operands and tables
   }
+  if (lookahead_inplace(algo, c, n, nb)) b += gemm_reserve(m, nb);  // side stream's split-K
   if (lookahead_layout(algo, c, n, nb)) {
     // the next block's buffers and the side stream's split-K region
     b += ((size_t)m * nb * 4 + 255) & ~(size_t)255;
@@ -314,7 +326,8 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
   const bool la_layout = lookahead_layout(algo, c, n, nb);
   void* plb[2] = {pl, la_layout ? ar.take((size_t)m * nb * 4) : nullptr};
   void* pub[2] = {pu, la_layout ? ar.take((size_t)nb * m * 4) : nullptr};
-  bool ahead = la_layout && h->lookahead;
+  const bool la_inplace = lookahead_inplace(algo, c, n, nb);
+  bool ahead = (la_layout || la_inplace) && h->lookahead;
   if (ahead) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(h->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
@@ -341,7 +354,7 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
   // +R) for the side stream (the arena's end is lowered while they run)
   const size_t ws_bytes0 = h->ws_bytes;
   const size_t side_lo = ar.off + gemm_reserve(m, nb), side_hi = side_lo + gemm_reserve(m, nb);
-  if (la_layout) h->ws_bytes = side_lo;
+  if (la_layout || la_inplace) h->ws_bytes = side_lo;
   const int64_t nc = (n - 1) / nb + 1;
   // outer update from the mirrors: C[rows r0.., cols c0..] (M x N) -= L[r0.., 0:K] U[0:K, c0..]
   // (K range [k0, k0 + K), k0 a multiple of 128; cin: C = cin - L U, cin
@@ -389,7 +402,21 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
     h->ws_used = side_lo;
     h->ws_bytes = side_hi;
     int st = 0;
-    if (K1 == 0) {
+    if (!buffered) {
+      // in place: block column and block row of block k1 minus blocks
+      // 0 .. k1-2 (nothing for k1 = 1)
+      const bool oc = W == REMIFA_R_32F && c.O != REMIFA_FP32;
+      const int64_t lo1 = oc ? ldc : ldda;
+      auto opnd = [&](int64_t i, int64_t j) {
+        return oc ? (const void*)at(cpy, REMIFA_R_16F, i, j, ldc) : (const void*)at(d_a, W, i, j, ldda);
+      };
+      if (K1 > 0) {
+        st = gemm(x, c.O, W, updm1, in1, K1, opnd(o1, 0), lo1, opnd(0, o1), lo1, blk1, ldda);
+        if (!st && updm1 > in1)
+          st = gemm(x, c.O, W, in1, updm1 - in1, K1, opnd(o1, 0), lo1, opnd(0, o1 + in1), lo1,
+                    at(d_a, W, o1, o1 + in1, ldda), ldda);
+      }
+    } else if (K1 == 0) {
       st = convert_launch(h, W, REMIFA_R_32F, updm1, in1, blk1, ldda, npl, ldpl);
       if (!st) st = convert_launch(h, W, REMIFA_R_32F, in1, updm1, blk1, ldda, npu, ldpu);
     } else if (fp8) {
@@ -437,6 +464,24 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
             RF_CHECK(gemm8(W, updm, in, ofs, ofs, ofs, blk, ldda));
             RF_CHECK(gemm8(W, in, updm - in, ofs, ofs, ofs + in, at(d_a, W, ofs, ofs + in, ldda),
                            ldda));
+          } else if (ahead) {
+            // the side stream subtracted blocks 0 .. kk-2 in place; block
+            // kk-1 (K = nb) follows here, then the side stream starts on
+            // block kk+1 while this block is factored
+            if (kk > 0) {
+              RF_HIP(hipStreamWaitEvent(h->stream, h->ev_side, 0));
+              const int64_t op = ofs - nb;
+              const void* lk = o_copy ? at(cpy, REMIFA_R_16F, ofs, op, ldc) : at(d_a, W, ofs, op, ldda);
+              const void* uk = o_copy ? at(cpy, REMIFA_R_16F, op, ofs, ldc) : at(d_a, W, op, ofs, ldda);
+              const void* uk2 = o_copy ? at(cpy, REMIFA_R_16F, op, ofs + in, ldc)
+                                       : at(d_a, W, op, ofs + in, ldda);
+              RF_CHECK(gemm(x, c.O, W, updm, in, nb, lk, lo, uk, lo, blk, ldda));
+              if (updm > in)
+                RF_CHECK(gemm(x, c.O, W, in, updm - in, nb, lk, lo, uk2, lo,
+                              at(d_a, W, ofs, ofs + in, ldda), ldda));
+              RF_HIP(hipEventRecord(h->ev_main, h->stream));
+            }
+            if (kk + 1 < nc) RF_CHECK(side_prep(kk + 1));
           } else if (ofs > 0) {
             // block column (:239-304) and block row (:311-410), in place
             RF_CHECK(gemm(x, c.O, W, updm, in, ofs, la, lo, ua, lo, blk, ldda));
