@@ -232,6 +232,7 @@ def test_gemm_every_tile_within_bound(handle, gpu, tile, ctype, m, n, k):
     (256, 256, 32, "f32", 0, "f16"),      # a single slab
     (512, 256, 128, "f16", 0, "f16"),     # fp16 output
     (1024, 512, 2048, "f32", 4, "f16"),   # split-K partials
+    (512, 256, 1024, "f16", 3, "f16"),    # ... three slices, fp16 output
     (512, 768, 512, "f32", 0, "bf16"),    # bf16 storage
 ])
 def test_gemm_large_tile_within_bound(handle, gpu, m, n, k, out, split, storage):
@@ -253,16 +254,24 @@ def test_gemm_large_tile_within_bound(handle, gpu, m, n, k, out, split, storage)
         ct, oc, opp = rf.TC32, orc.TC32, orc.F16
     if out == "f16":
         c = orc.round_to(orc.F16, c)
-    dc = to_dev(orc.f32_to_f16_bits(c) if out == "f16" else c, gpu, ld=m + 4)
+    c_dev = orc.f32_to_f16_bits(c) if out == "f16" else c
+    runs = []
     handle.set_gemm_tile(4)
     handle.set_tuning(0, split)
     try:
-        rf.gemmex(handle, ct, da, db, dc, alpha=-1.0, beta=1.0)
-        torch.cuda.synchronize()
+        # the split-K partials are added in slice order: a second run gives
+        # the same bits
+        for _ in range(2 if split > 1 else 1):
+            dc = to_dev(c_dev, gpu, ld=m + 4)
+            rf.gemmex(handle, ct, da, db, dc, alpha=-1.0, beta=1.0)
+            torch.cuda.synchronize()
+            runs.append(to_np(dc))
     finally:
         handle.set_gemm_tile(-1)
         handle.set_tuning(0, 0)
-    got = to_np(dc)
+    if len(runs) == 2:
+        assert runs[0].tobytes() == runs[1].tobytes()
+    got = runs[0]
     got = orc.f16_bits_to_f32(got) if out == "f16" else got
     ref = orc.gemm_sub(oc, a, b, c, orc.F16 if out == "f16" else orc.F32)
     aa, bb = orc.round_to(opp, a).astype(np.float64), orc.round_to(opp, b).astype(np.float64)
