@@ -36,6 +36,20 @@
 // diagonal block and its own earlier results: everything it reads was
 // written by the diagonal kernel before the launch or by itself.
 //
+// One-launch form (the default): both in one grid. The first workgroup to
+// start (an atomic ticket, not blockIdx) takes the diagonal role: per slice
+// the tile's update, factorization and table, published through a counter,
+// then only the next tile's four units. The other workgroups (helpers) start
+// slice s when the counter says so and take every other unit: the far units
+// and the diagonal block's rows / columns beyond the next tile, whose
+// progress they publish per 32-vector block; the diagonal role waits for
+// that before it takes such a block as the next tile's. A helper's slice s
+// reads only the table and the tile's vectors, all final by the time slice
+// s is published. Helpers wait only for the diagonal role, which waits only
+// for slices that helpers can finish without further waits; the grid stays
+// below half the CUs, so a helper that has not started finds a CU once the
+// other stream's workgroups there end.
+//
 // MFMA operand layout used below (v_mfma_f32_32x32x16_f16, D = A B + C):
 // lane l supplies A[i = l % 32][k = 8 (l / 32) .. +7] and
 // B[k = 8 (l / 32) .. +7][j = l % 32]; register r of lane l holds
@@ -90,6 +104,14 @@ struct BlkArgs {
   half_t* uop;          // op(U) columns, k-contiguous: uop[v * ldo + k]
   int64_t ldo;
   SliceTab* tabs;       // one per slice
+  // one-launch form: sync[0] ticket counter (tickets of this launch start at
+  // tbase), sync[1] published slices (epoch * 16 + s + 1 once slice s's
+  // table is written; epoch = launch number within the factorization),
+  // sync[2] wait-limit flag, sync[4 + 16 role + b] the slices done of
+  // diagonal-block unit b (same encoding)
+  int* sync;
+  int tbase;
+  int epoch;
 };
 
 // LDS of the diagonal kernel (one workgroup; 162,048 bytes of the CU's 160 KiB)
@@ -288,7 +310,9 @@ __device__ __forceinline__ void stage_panel(_Float16 (*pan)[PP], const half_t* s
   }
 }
 
-template <int MODE>
+constexpr long long kWaitLimit = 1LL << 21;  // polls of a published counter per wait
+
+template <int MODE, bool PUB>
 __device__ void diagonal_role(const BlkArgs& g, DiagLds& S) {
   using Sm = Sem<MODE>;
   const int t = threadIdx.x;
@@ -373,17 +397,48 @@ __device__ void diagonal_role(const BlkArgs& g, DiagLds& S) {
         put_o(1, i - 32, c, x[jj]);
       }
     }
+    // (publish: every wavefront's stores have reached the L2 before the
+    // barrier; the release store after it writes the L2 back once)
+    if constexpr (PUB) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
+    if constexpr (PUB)
+      if (t == 0)
+        __hip_atomic_store(g.sync + 1, g.epoch * 16 + s + 1, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
     // 4. the units inside the diagonal block, rows of L and columns of U
     // interleaved, the next tile's rows and columns (the first unit pair of
-    // each kind) first
+    // each kind) first. In the one-launch form only those four: the
+    // helpers take the rest (blocks of 32 vectors from 2 s + 4 on) and
+    // publish each block's slices, which the next-tile units wait for.
     const int nun = (g.nb - so - SW) / NV;  // units of each kind (even)
-    for (int q = w; q < 2 * nun; q += BLT / 64) {
+    const int nq = PUB ? std::min(4, 2 * nun) : 2 * nun;
+    for (int q = w; q < nq; q += BLT / 64) {
       int role, u;
       if (q < 4) {
         role = q >> 1, u = q & 1;
       } else {
         role = q & 1, u = 2 + ((q - 4) >> 1);
+      }
+      if constexpr (PUB) {
+        if (s > 0) {
+          // block 2 s + 2 + u: its slices 0 .. s-1 came from a helper
+          const int want = g.epoch * 16 + s;
+          int* prog = g.sync + 4 + 16 * role + 2 * s + 2 + u;
+          if (i == 0) {
+            for (long long it = 0;; ++it) {
+              if (__hip_atomic_load(prog, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= want)
+                break;
+              if (it > kWaitLimit ||
+                  ((it & 255) == 0 &&
+                   __hip_atomic_load(g.sync + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+                __hip_atomic_store(g.sync + 2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+              }
+              __builtin_amdgcn_s_sleep(2);
+            }
+          }
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
       }
       do_unit<MODE, 14>(g, S.tab.role[role], S.pan[role], role, so + SW + NV * u, so);
     }
@@ -394,7 +449,7 @@ __device__ void diagonal_role(const BlkArgs& g, DiagLds& S) {
 template <int MODE>
 __global__ void __launch_bounds__(BLT) blockll_diag_kernel(BlkArgs g) {
   __shared__ DiagLds S;
-  diagonal_role<MODE>(g, S);
+  diagonal_role<MODE, false>(g, S);
 }
 
 // far workgroups 0 .. nfw-1 take rows of L, the others columns of U
@@ -429,13 +484,104 @@ blockll_far_kernel(BlkArgs g) {
   far_body<MODE, KC>(g);
 }
 
+// One-launch form, far role: helper hx of nhelp; its wavefronts take units
+// q = (global wavefront) + j * (helper wavefronts) through every slice (both
+// roles' tables and panels staged per slice): first the diagonal block's
+// blocks 4 .. nb/32 - 1 of each role (role q % 2, block 4 + q / 2; block b
+// while it is below the next tile, slices s <= b/2 - 2, its progress
+// published after each), then the far units (role f % 2, unit f / 2 for
+// f = q - 2 ndu).
+template <int MODE>
+__device__ void helper_role(const BlkArgs& g, DiagLds& S, int hx, int nhelp) {
+  const int t = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(t / 64);
+  const int ns = g.nb / SW;
+  const int hw = nhelp * (BLT / 64), gw = hx * (BLT / 64) + w;
+  const int ndu = std::max(0, g.nb / NV - 4);  // diagonal-block units per role
+  const int tot = 2 * ndu + 2 * g.nfu;
+  bool gave_up = false;
+  for (int s = 0; s < ns; ++s) {
+    const int so = SW * s;
+    if (t == 0 && !gave_up) {
+      const int want = g.epoch * 16 + s + 1;
+      for (long long it = 0;; ++it) {
+        if (__hip_atomic_load(g.sync + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= want)
+          break;
+        if (it > kWaitLimit ||
+            ((it & 255) == 255 &&
+             __hip_atomic_load(g.sync + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+          // give up (the host reports the error and the results are void)
+          __hip_atomic_store(g.sync + 2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          gave_up = true;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(4);
+      }
+      // (one invalidation of this CU's caches for the whole workgroup)
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    {
+      const float4* src = (const float4*)&g.tabs[s];
+      float4* dst = (float4*)&S.tab;
+      for (int e = t; e < (int)(sizeof(SliceTab) / 16); e += BLT) dst[e] = src[e];
+    }
+    stage_panel(S.pan[0], g.uop, g.ldo, so);
+    stage_panel(S.pan[1], g.lop, g.ldo, so);
+    __syncthreads();
+    for (int q = gw; q < tot; q += hw) {
+      if (q < 2 * ndu) {
+        const int role = q & 1, b = 4 + (q >> 1);
+        if (2 * s + 4 > b) continue;  // the diagonal role's from here on
+        do_unit<MODE, 8>(g, S.tab.role[role], S.pan[role], role, (int64_t)NV * b, so);
+        // (this wavefront's stores have reached the L2; the release store
+        // writes it back)
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        if ((t & 63) == 0)
+          __hip_atomic_store(g.sync + 4 + 16 * role + b, g.epoch * 16 + s + 1, __ATOMIC_RELEASE,
+                             __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        const int f = q - 2 * ndu, role = f & 1;
+        do_unit<MODE, 8>(g, S.tab.role[role], S.pan[role], role, g.nb + (int64_t)NV * (f >> 1),
+                         so);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(BLT) blockll_fused_kernel(BlkArgs g, int nhelp) {
+  __shared__ DiagLds S;
+  __shared__ int s_tk;
+  if (threadIdx.x == 0)
+    s_tk = __hip_atomic_fetch_add(g.sync, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - g.tbase;
+  __syncthreads();
+  const int tk = s_tk;
+  if (tk == 0)
+    diagonal_role<MODE, true>(g, S);
+  else if (tk > 0 && tk <= nhelp)  // (any other ticket: counter not reset; the waits report it)
+    helper_role<MODE>(g, S, tk - 1, nhelp);
+}
+
+__global__ void blockll_sync_reset_kernel(int* sync) {
+  sync[threadIdx.x] = 0;  // (64 words)
+}
+
 }  // namespace
 
 // Workspace of the block kernels for an m x m matrix and outer blocks of
-// nb: op(L) rows and op(U) columns (m x nb fp16 each) and the slice tables.
+// nb: op(L) rows and op(U) columns (m x nb fp16 each), the slice tables and
+// the one-launch form's sync words.
 size_t blockll_workspace_bytes(int64_t m, int64_t nb) {
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  return 2 * al((size_t)m * nb * 2) + al((size_t)(nb / SW) * sizeof(SliceTab));
+  return 2 * al((size_t)m * nb * 2) + al((size_t)(nb / SW) * sizeof(SliceTab)) + 256;
+}
+
+// the sync words inside that workspace
+int* blockll_sync_words(void* base, int64_t m, int64_t nb) {
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  return (int*)((char*)base + 2 * al((size_t)m * nb * 2) + al((size_t)(nb / SW) * sizeof(SliceTab)));
 }
 
 // whether an outer block of width in (outer blocks of nb) takes the block
@@ -448,9 +594,11 @@ bool blockll_eligible(int ib, int I, int64_t in, int64_t nb, int64_t m, int64_t 
 
 // Outer block kk: pl / pu the fp32 buffers, a the fp16 factors at the
 // block's origin, in = block width (<= nb), updm = rows of the block column.
+// seq: {launches so far, tickets so far} of this factorization's one-launch
+// calls (both 0 before the first: its sync words are cleared then).
 int blockll_launch(remifa_handle_t h, void* base, int64_t m, int64_t nb, int64_t updm,
                    int64_t in, const float* pl, int64_t ldpl, const float* pu, int64_t ldpu,
-                   half_t* a, int64_t lda) {
+                   half_t* a, int64_t lda, int* seq) {
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   char* p = (char*)base;
   BlkArgs g;
@@ -471,6 +619,29 @@ int blockll_launch(remifa_handle_t h, void* base, int64_t m, int64_t nb, int64_t
   // far workgroups of each role: one unit per wavefront and slice
   g.nfw = (g.nfu + BLT / 64 - 1) / (BLT / 64);
   const double bytes = 2.0 * (double)updm * in * (4 + 2 + 2);
+  g.sync = blockll_sync_words(base, m, nb);
+  if (h->blk_fused) {
+    // helpers: about blk_upw units per wavefront (automatic: 3, the fastest
+    // of 1 .. 6 for the n = 32768 factorization with the second stream)
+    const int upw = h->blk_upw > 0 ? h->blk_upw : 3;
+    // (the grid stays below half the CUs: the diagonal role's waits on
+    // helpers that have not started end once a CU frees up)
+    const int64_t tot = 2 * (int64_t)std::max<int64_t>(0, in / NV - 4) + 2 * (int64_t)g.nfu;
+    int64_t nh = ceil_div(tot, (int64_t)upw * (BLT / 64));
+    nh = std::min<int64_t>(nh, std::max(1, h->num_cu / 2 - 1));
+    const int nhelp = (int)nh;
+    if (seq[0] == 0) blockll_sync_reset_kernel<<<1, 64, 0, h->stream>>>(g.sync);
+    g.epoch = ++seq[0];
+    g.tbase = seq[1];
+    seq[1] += 1 + nhelp;
+    timer_begin(h, KC_PANEL, bytes, 4);
+    blockll_fused_kernel<SM_F32_TC><<<1 + nhelp, BLT, 0, h->stream>>>(g, nhelp);
+    RF_HIP(hipGetLastError());
+    timer_end(h);
+    return 0;
+  }
+  g.tbase = 0;
+  g.epoch = 0;
   timer_begin(h, KC_PANEL, bytes, 2);
   blockll_diag_kernel<SM_F32_TC><<<1, BLT, 0, h->stream>>>(g);
   RF_HIP(hipGetLastError());

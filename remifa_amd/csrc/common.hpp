@@ -74,6 +74,11 @@ struct remifa_handle_st {
   int panel_width = 0;
   int split_k = 0;
   int gemm_tile = -1;  // -1: chosen per shape
+  // block kernels of the LL inner factorization: one launch (1) or the
+  // diagonal and far kernels one after the other (0); far units per
+  // helper wavefront of the one-launch form (0 = automatic)
+  int blk_fused = 1;
+  int blk_upw = 0;
   // LL lookahead: the next outer block's update runs on a second stream
   // while the current block is factored (1 = on, 0 = off)
   int lookahead = 1;

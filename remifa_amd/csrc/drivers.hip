@@ -32,10 +32,11 @@ int subpanel_launch(remifa_handle_t h, int T, int I, bool rl, int ib, int64_t cm
                     int64_t ldl, void* u, int64_t ldu, half_t* cp, int64_t ldcp, float* tab);
 size_t subpanel_table_bytes();
 size_t blockll_workspace_bytes(int64_t m, int64_t nb);
+int* blockll_sync_words(void* base, int64_t m, int64_t nb);
 bool blockll_eligible(int ib, int I, int64_t in, int64_t nb, int64_t m, int64_t lda);
 int blockll_launch(remifa_handle_t h, void* base, int64_t m, int64_t nb, int64_t updm,
                    int64_t in, const float* pl, int64_t ldpl, const float* pu, int64_t ldpu,
-                   half_t* a, int64_t lda);
+                   half_t* a, int64_t lda, int* seq);
 
 namespace {
 
@@ -321,6 +322,7 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
   void* blkws = buffered && h->panel_width == 0 && blockll_eligible((int)c.ib, c.I, nb, nb, m, 4)
                     ? ar.take(blockll_workspace_bytes(m, nb))
                     : nullptr;
+  int blk_seq[2] = {0, 0};  // one-launch block kernels: launches, tickets
   // lookahead: block kk is factored in buffers plb/pub[kk % 2] while the
   // side stream prepares block kk + 1 in the other pair
   const bool la_layout = lookahead_layout(algo, c, n, nb);
@@ -559,7 +561,7 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
           // so the rounded factors land in A as they are produced
           if (blkws && blockll_eligible((int)c.ib, c.I, in, nb, m, ldda))
             RF_CHECK(blockll_launch(h, blkws, m, nb, updm, in, (const float*)cpl, ldpl,
-                                    (const float*)cpu, ldpu, (half_t*)blk, ldda));
+                                    (const float*)cpu, ldpu, (half_t*)blk, ldda, blk_seq));
           else
             RF_CHECK(inner_ll(x, REMIFA_R_32F, updm, in, cpl, ldpl, cpu, ldpu, blk, ldda, true));
           if (fp8) RF_CHECK(mirror8(ofs, in));
@@ -639,7 +641,15 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
     RF_HIP(hipGetLastError());
   }
   // the reference drivers synchronise before returning (:567-569)
+  int waited_out = 0;
+  if (blk_seq[0] > 0)
+    RF_HIP(hipMemcpyAsync(&waited_out, blockll_sync_words(blkws, m, nb) + 2, sizeof(int),
+                          hipMemcpyDeviceToHost, h->stream));
   RF_HIP(hipStreamSynchronize(h->stream));
+  if (waited_out) {
+    set_last_error("block kernels: wait for the diagonal role timed out");
+    return REMIFA_ERR_HIP;
+  }
   return 0;
 }
 

@@ -142,6 +142,15 @@ int remifa_set_tuning(remifa_handle_t h, int panel_width, int split_k) {
   return 0;
 }
 
+int remifa_set_block_kernels(remifa_handle_t h, int one_launch, int units_per_wave) {
+  if (!h) return -1;
+  if (one_launch != 0 && one_launch != 1) return -2;
+  if (units_per_wave < 0 || units_per_wave > 64) return -3;
+  h->blk_fused = one_launch;
+  h->blk_upw = units_per_wave;
+  return 0;
+}
+
 int remifa_set_gemm_rules(remifa_handle_t h, int count, const int64_t* rules) {
   if (!h) return -1;
   if (count < 0 || count > 4096) return -2;

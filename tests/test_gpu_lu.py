@@ -335,6 +335,46 @@ def test_block_kernel_matches_slices_and_oracle(handle, gpu, n, nb, ld_pad):
     assert_parity(out[0], out[64], f"block kernel vs slice kernels n={n} nb={nb}")
 
 
+@pytest.mark.parametrize("n,nb,ld_pad,upw,lookahead,oracle",
+                         [(1536, 512, 0, 0, True, True), (1000, 256, 4, 1, True, True),
+                          (640, 192, 8, 3, False, True), (777, 128, 0, 64, True, True),
+                          (4096, 512, 0, 0, True, False), (3000, 512, 0, 5, True, False)])
+def test_block_kernels_one_launch_bit_identical(handle, gpu, n, nb, ld_pad, upw, lookahead,
+                                                oracle):
+    """The block kernels' one-launch form (the diagonal role publishing each
+    slice, the far units following it in the same grid) against the
+    two-launch form: bit-identical factors, with the helper count varied
+    (far units per wavefront 1 .. 64; 0 = automatic), ragged blocks and
+    units cut by the matrix edge; against the oracle where it finishes in
+    seconds."""
+    import torch
+    rf = _rf()
+    a32 = orc.gen_diagdom_alpha(n, 13 + n, 1.1)
+    a16 = orc.f32_to_f16_bits(a32)
+    cfg = rf.api.make_config(W=rf.R_16F, I=rf.TC32, O=rf.TC32, OP=rf.R_32F, ib=32)
+    out = {}
+    try:
+        handle.set_lookahead(lookahead, 0)
+        for one in (True, False, True):
+            handle.set_block_kernels(one, upw)
+            da = to_dev(a16, gpu, ld=n + ld_pad)
+            assert rf.lu_nopiv(handle, da, nb, algo=rf.LL, cfg=cfg) == 0
+            torch.cuda.synchronize()
+            f = to_np(da)
+            if one in out:  # a second one-launch run: the same bits again
+                assert f.tobytes() == out[one].tobytes(), f"one-launch rerun differs n={n}"
+            out[one] = f
+    finally:
+        handle.set_block_kernels(True, 0)
+        handle.set_lookahead(True, 0)
+    assert out[True].tobytes() == out[False].tobytes(), f"one vs two launches differ n={n} nb={nb}"
+    if oracle:
+        o = orc.lu_nopiv("ll", orc.cfg(W=orc.F16, I=orc.TC32, O=orc.TC32, OP=orc.F32, ib=32),
+                         a16, nb)
+        f64 = orc.lu_f64(orc.round_to(orc.F16, a32))
+        _check(out[True], o, f64, orc.F16, f"one-launch block kernels n={n} nb={nb}")
+
+
 @pytest.mark.parametrize("n,nb,ld_pad,lookahead", [(1536, 512, 0, True), (2048, 256, 8, True),
                                                    (1536, 512, 0, False), (1280, 256, 0, True)])
 def test_large_tile_updates_match_oracle(handle, gpu, n, nb, ld_pad, lookahead):
