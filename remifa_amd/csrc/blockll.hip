@@ -299,6 +299,184 @@ __device__ __forceinline__ void do_unit(const BlkArgs& g, const RoleTab& T,
   }
 }
 
+// solve_block for N units of one role at once (the same table rows feed
+// all of them; their pivot chains interleave)
+template <int MODE, int N>
+__device__ __forceinline__ void solve_blocks(float (&x)[N][16], const float (*m)[32], int lh) {
+  using Sm = Sem<MODE>;
+  constexpr int PF = 3;  // row buffers in flight
+  float4 mv[PF][4];
+  auto fetch = [&](int k) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (8 * q + 7 >= k) mv[k % PF][q] = *(const float4*)&m[k][8 * q + 4 * lh];
+  };
+  fetch(0);
+  fetch(1);
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    asm volatile("" ::: "memory");
+    if (k + 2 < 32) fetch(k + 2);
+    const int hk = (k >> 2) & 1, ik = 4 * (k >> 3) + (k & 3);
+    const float4 d4 = mv[k % PF][k >> 3];
+    const float dk[4] = {d4.x, d4.y, d4.z, d4.w};
+    float lk[N];
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      const float own = Sm::mul(x[n][ik], dk[k & 3]);
+      lk[n] = hk ? bcast_half<1>(own) : bcast_half<0>(own);
+      x[n][ik] = lh == hk ? lk[n] : x[n][ik];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (8 * q + 7 <= k) continue;
+      const float4 v4 = mv[k % PF][q];
+      const float mm[4] = {v4.x, v4.y, v4.z, v4.w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int e0 = 8 * q + j, e1 = e0 + 4;
+        if (e1 <= k) continue;
+#pragma unroll
+        for (int n = 0; n < N; ++n) {
+          const float y = Sm::fnma(lk[n], mm[j], x[n][4 * q + j]);
+          x[n][4 * q + j] = (e0 > k || lh) ? y : x[n][4 * q + j];
+        }
+      }
+    }
+  }
+}
+
+// do_unit for N units of one role at once (vectors from v0[n]): the same
+// arithmetic per unit; the other factor's tile vectors and the table rows
+// are read once for all of them and their chains interleave.
+template <int MODE, int KC, int N>
+__device__ __forceinline__ void do_units(const BlkArgs& g, const RoleTab& T,
+                                         const _Float16 (*pan)[PP], int role,
+                                         const int64_t (&v0)[N], int so) {
+  const int lane = threadIdx.x & 63, l32 = lane & 31, lh = lane >> 5;
+  const int64_t ldo = g.ldo;
+  bool ok[N];
+  int64_t vc[N];
+  const half_t* own[N];
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    const int64_t v = v0[n] + l32;
+    ok[n] = v < g.updm;
+    vc[n] = ok[n] ? v : g.updm - 1;
+    own[n] = (role ? g.uop : g.lop) + vc[n] * ldo + 8 * lh;
+  }
+  bf16v acc0[N], acc1[N];
+  if (role == 0) {
+    const auto rc = __builtin_amdgcn_make_buffer_rsrc((void*)(g.pl + (int64_t)so * g.ldpl), 0,
+                                                      (int)(SW * g.ldpl * 4), kBufWord3);
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      const int vo = (int)(vc[n] + 4 * lh * g.ldpl) * 4;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int eu = 8 * (r >> 2) + (r & 3);
+        acc0[n][r] = -u2f(__builtin_amdgcn_raw_buffer_load_b32(rc, vo, (int)(eu * g.ldpl * 4), 0));
+        acc1[n][r] =
+            -u2f(__builtin_amdgcn_raw_buffer_load_b32(rc, vo, (int)((32 + eu) * g.ldpl * 4), 0));
+      }
+    }
+  } else {
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      const float* c = g.pu + so + vc[n] * g.ldpu + 4 * lh;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float4 p = *(const float4*)(c + 8 * q);
+        const float4 sv = *(const float4*)(c + 32 + 8 * q);
+        acc0[n][4 * q] = -p.x, acc0[n][4 * q + 1] = -p.y, acc0[n][4 * q + 2] = -p.z,
+                  acc0[n][4 * q + 3] = -p.w;
+        acc1[n][4 * q] = -sv.x, acc1[n][4 * q + 1] = -sv.y, acc1[n][4 * q + 2] = -sv.z,
+                  acc1[n][4 * q + 3] = -sv.w;
+      }
+    }
+  }
+  for (int c0 = 0; c0 < so; c0 += 16 * KC) {
+    bh8 y[N][KC];
+#pragma unroll
+    for (int n = 0; n < N; ++n)
+#pragma unroll
+      for (int u = 0; u < KC; ++u)
+        if (c0 + 16 * u < so) y[n][u] = *(const bh8*)(own[n] + c0 + 16 * u);
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      if (u % 2 == 0) asm volatile("" ::: "memory");
+      if (c0 + 16 * u < so) {
+        const bh8 a0 = *(const bh8*)&pan[l32][c0 + 16 * u + 8 * lh];
+        const bh8 a1 = *(const bh8*)&pan[32 + l32][c0 + 16 * u + 8 * lh];
+#pragma unroll
+        for (int n = 0; n < N; ++n) {
+          acc0[n] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a0, y[n][u], acc0[n], 0, 0, 0);
+          acc1[n] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a1, y[n][u], acc1[n], 0, 0, 0);
+        }
+      }
+    }
+  }
+  float x0[N][16], x1[N][16];
+#pragma unroll
+  for (int n = 0; n < N; ++n)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) x0[n][r] = -acc0[n][r];
+  solve_blocks<MODE, N>(x0, T.m[0], lh);
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks) {
+    const int g0 = 8 * ks, g1 = 8 * ks + 4;
+    const bh8 av = *(const bh8*)&T.mot[l32][16 * ks + 8 * lh];
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      const unsigned p0 = pack_h2(x0[n][g0], x0[n][g0 + 1]), p1 = pack_h2(x0[n][g0 + 2], x0[n][g0 + 3]);
+      const unsigned q0 = pack_h2(x0[n][g1], x0[n][g1 + 1]), q1 = pack_h2(x0[n][g1 + 2], x0[n][g1 + 3]);
+      const auto sa = __builtin_amdgcn_permlane32_swap(p0, q0, false, false);
+      const auto sb = __builtin_amdgcn_permlane32_swap(p1, q1, false, false);
+      typedef unsigned u4 __attribute__((ext_vector_type(4)));
+      const u4 yb = {sa[0], sb[0], sa[1], sb[1]};
+      acc1[n] = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, __builtin_bit_cast(bh8, yb), acc1[n], 0,
+                                                       0, 0);
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < N; ++n)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) x1[n][r] = -acc1[n][r];
+  solve_blocks<MODE, N>(x1, T.m[1], lh);
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    if (!ok[n]) continue;
+    half_t* oc = (role ? g.uop : g.lop) + vc[n] * ldo + so + 4 * lh;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const bh4 h0 = {(_Float16)x0[n][4 * q], (_Float16)x0[n][4 * q + 1], (_Float16)x0[n][4 * q + 2],
+                      (_Float16)x0[n][4 * q + 3]};
+      const bh4 h1 = {(_Float16)x1[n][4 * q], (_Float16)x1[n][4 * q + 1], (_Float16)x1[n][4 * q + 2],
+                      (_Float16)x1[n][4 * q + 3]};
+      *(bh4*)(oc + 8 * q) = h0;
+      *(bh4*)(oc + 32 + 8 * q) = h1;
+      if (role) {
+        half_t* ac = g.a + so + vc[n] * g.lda + 4 * lh;
+        *(bh4*)(ac + 8 * q) = h0;
+        *(bh4*)(ac + 32 + 8 * q) = h1;
+      }
+    }
+    if (!role) {
+      const auto ra = __builtin_amdgcn_make_buffer_rsrc((void*)(g.a + (int64_t)so * g.lda), 0,
+                                                        (int)(SW * g.lda * 2), kBufWord3);
+      const int vo = (int)(vc[n] + 4 * lh * g.lda) * 2;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int eu = 8 * (r >> 2) + (r & 3);
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, (_Float16)x0[n][r]),
+                                              ra, vo, (int)(eu * g.lda * 2), 0);
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, (_Float16)x1[n][r]),
+                                              ra, vo, (int)((32 + eu) * g.lda * 2), 0);
+      }
+    }
+  }
+}
+
 // the other factor's op-rounded vectors of the slice's tile into LDS:
 // pan[e][k] = src[(so + e) * ldo + k], e < 64, k < so
 __device__ __forceinline__ void stage_panel(_Float16 (*pan)[PP], const half_t* src, int64_t ldo,
@@ -529,19 +707,25 @@ __device__ void helper_role(const BlkArgs& g, DiagLds& S, int hx, int nhelp) {
     stage_panel(S.pan[0], g.uop, g.ldo, so);
     stage_panel(S.pan[1], g.lop, g.ldo, so);
     __syncthreads();
-    for (int q = gw; q < tot; q += hw) {
-      if (q < 2 * ndu) {
-        const int role = q & 1, b = 4 + (q >> 1);
-        if (2 * s + 4 > b) continue;  // the diagonal role's from here on
-        do_unit<MODE, 8>(g, S.tab.role[role], S.pan[role], role, (int64_t)NV * b, so);
-        // (this wavefront's stores have reached the L2; the release store
-        // writes it back)
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        if ((t & 63) == 0)
-          __hip_atomic_store(g.sync + 4 + 16 * role + b, g.epoch * 16 + s + 1, __ATOMIC_RELEASE,
-                             __HIP_MEMORY_SCOPE_AGENT);
+    int q = gw;
+    for (; q < tot && q < 2 * ndu; q += hw) {
+      const int role = q & 1, b = 4 + (q >> 1);
+      if (2 * s + 4 > b) continue;  // the diagonal role's from here on
+      do_unit<MODE, 8>(g, S.tab.role[role], S.pan[role], role, (int64_t)NV * b, so);
+      // (this wavefront's stores have reached the L2; the release store
+      // writes it back)
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      if ((t & 63) == 0)
+        __hip_atomic_store(g.sync + 4 + 16 * role + b, g.epoch * 16 + s + 1, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // far units two at a time (q and q + hw: hw is even, so the same role)
+    for (; q < tot; q += 2 * hw) {
+      const int f = q - 2 * ndu, role = f & 1;
+      if (q + hw < tot) {
+        const int64_t v0[2] = {g.nb + (int64_t)NV * (f >> 1), g.nb + (int64_t)NV * ((f + hw) >> 1)};
+        do_units<MODE, 4, 2>(g, S.tab.role[role], S.pan[role], role, v0, so);
       } else {
-        const int f = q - 2 * ndu, role = f & 1;
         do_unit<MODE, 8>(g, S.tab.role[role], S.pan[role], role, g.nb + (int64_t)NV * (f >> 1),
                          so);
       }
