@@ -776,6 +776,27 @@ bool blockll_eligible(int ib, int I, int64_t in, int64_t nb, int64_t m, int64_t 
          lda % 4 == 0 && (int64_t)SW * std::max(m, lda) * 4 < (1LL << 31);
 }
 
+// Helper workgroups of the one-launch form for an outer block of width in
+// with updm rows: about blk_upw units per wavefront (automatic: 3, the
+// fastest of 1 .. 8 for the n = 32768 factorization with the second
+// stream). The grid stays below half the CUs: the diagonal role's waits on
+// helpers that have not started end once a CU frees up.
+int blockll_helpers(remifa_handle_t h, int64_t updm, int64_t in) {
+  const int upw = h->blk_upw > 0 ? h->blk_upw : 3;
+  const int64_t far = updm - in;
+  const int64_t nfu = far > 0 ? (far + NV - 1) / NV : 0;
+  const int64_t tot = 2 * std::max<int64_t>(0, in / NV - 4) + 2 * nfu;
+  const int64_t nh = ceil_div(tot, (int64_t)upw * (BLT / 64));
+  return (int)std::min<int64_t>(nh, std::max(1, h->num_cu / 2 - 1));
+}
+
+// CUs the block kernels hold for an outer block while they run: the
+// one-launch form's workgroups (161 KB of LDS each, one per CU); 0 for the
+// two-launch form (one workgroup, then a short far kernel)
+int blockll_cus(remifa_handle_t h, int64_t updm, int64_t in) {
+  return h->blk_fused ? 1 + blockll_helpers(h, updm, in) : 0;
+}
+
 // Outer block kk: pl / pu the fp32 buffers, a the fp16 factors at the
 // block's origin, in = block width (<= nb), updm = rows of the block column.
 // seq: {launches so far, tickets so far} of this factorization's one-launch
@@ -805,15 +826,7 @@ int blockll_launch(remifa_handle_t h, void* base, int64_t m, int64_t nb, int64_t
   const double bytes = 2.0 * (double)updm * in * (4 + 2 + 2);
   g.sync = blockll_sync_words(base, m, nb);
   if (h->blk_fused) {
-    // helpers: about blk_upw units per wavefront (automatic: 3, the fastest
-    // of 1 .. 6 for the n = 32768 factorization with the second stream)
-    const int upw = h->blk_upw > 0 ? h->blk_upw : 3;
-    // (the grid stays below half the CUs: the diagonal role's waits on
-    // helpers that have not started end once a CU frees up)
-    const int64_t tot = 2 * (int64_t)std::max<int64_t>(0, in / NV - 4) + 2 * (int64_t)g.nfu;
-    int64_t nh = ceil_div(tot, (int64_t)upw * (BLT / 64));
-    nh = std::min<int64_t>(nh, std::max(1, h->num_cu / 2 - 1));
-    const int nhelp = (int)nh;
+    const int nhelp = blockll_helpers(h, updm, in);
     if (seq[0] == 0) blockll_sync_reset_kernel<<<1, 64, 0, h->stream>>>(g.sync);
     g.epoch = ++seq[0];
     g.tbase = seq[1];

@@ -79,6 +79,9 @@ struct remifa_handle_st {
   // helper wavefront of the one-launch form (0 = automatic)
   int blk_fused = 1;
   int blk_upw = 0;
+  // CUs that concurrent work of the same driver holds while the next update
+  // GEMMs run (the split-K model plans their rounds on the rest)
+  int cu_busy = 0;
   // LL lookahead: the next outer block's update runs on a second stream
   // while the current block is factored (1 = on, 0 = off)
   int lookahead = 1;

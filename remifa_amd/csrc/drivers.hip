@@ -33,6 +33,7 @@ int subpanel_launch(remifa_handle_t h, int T, int I, bool rl, int ib, int64_t cm
 size_t subpanel_table_bytes();
 size_t blockll_workspace_bytes(int64_t m, int64_t nb);
 int* blockll_sync_words(void* base, int64_t m, int64_t nb);
+int blockll_cus(remifa_handle_t h, int64_t updm, int64_t in);
 bool blockll_eligible(int ib, int I, int64_t in, int64_t nb, int64_t m, int64_t lda);
 int blockll_launch(remifa_handle_t h, void* base, int64_t m, int64_t nb, int64_t updm,
                    int64_t in, const float* pl, int64_t ldpl, const float* pu, int64_t ldpu,
@@ -421,19 +422,30 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
     } else if (K1 == 0) {
       st = convert_launch(h, W, REMIFA_R_32F, updm1, in1, blk1, ldda, npl, ldpl);
       if (!st) st = convert_launch(h, W, REMIFA_R_32F, in1, updm1, blk1, ldda, npu, ldpu);
-    } else if (fp8) {
-      // buffer = A (fp16, read by the epilogue) - L U from the e4m3 mirrors
-      st = gemm8(REMIFA_R_32F, updm1, in1, K1, o1, o1, npl, ldpl, 0, blk1);
-      if (!st) st = gemm8(REMIFA_R_32F, in1, updm1, K1, o1, o1, npu, ldpu, 0, blk1);
     } else {
-      const int oin = operand_type(c.O);
-      const void* l1 = at(obase, otyp, o1, 0, old_);
-      const void* u1 = at(obase, otyp, 0, o1, old_);
-      st = gemm_launch_cin(h, c.O, bf16m ? REMIFA_R_16BF : oin, REMIFA_R_32F, updm1, in1, K1,
-                           -1.0f, l1, old_, u1, old_, 1.0f, blk1, W, ldda, npl, ldpl);
-      if (!st)
-        st = gemm_launch_cin(h, c.O, bf16m ? REMIFA_R_16BF : oin, REMIFA_R_32F, in1, updm1, K1,
-                             -1.0f, l1, old_, u1, old_, 1.0f, blk1, W, ldda, npu, ldpu);
+      // the block column's update starts with block k1-1's inner
+      // factorization on the handle's stream: its split-K rounds are
+      // planned on the CUs the block kernels leave (the block row's update
+      // follows, mostly after it)
+      const int64_t ok0 = o1 - nb;
+      if (blkws && blockll_eligible((int)c.ib, c.I, std::min(n - ok0, nb), nb, m, ldda))
+        h->cu_busy = blockll_cus(h, m - ok0, std::min(n - ok0, nb));
+      if (fp8) {
+        // buffer = A (fp16, read by the epilogue) - L U from the e4m3 mirrors
+        st = gemm8(REMIFA_R_32F, updm1, in1, K1, o1, o1, npl, ldpl, 0, blk1);
+        h->cu_busy = 0;
+        if (!st) st = gemm8(REMIFA_R_32F, in1, updm1, K1, o1, o1, npu, ldpu, 0, blk1);
+      } else {
+        const int oin = operand_type(c.O);
+        const void* l1 = at(obase, otyp, o1, 0, old_);
+        const void* u1 = at(obase, otyp, 0, o1, old_);
+        st = gemm_launch_cin(h, c.O, bf16m ? REMIFA_R_16BF : oin, REMIFA_R_32F, updm1, in1, K1,
+                             -1.0f, l1, old_, u1, old_, 1.0f, blk1, W, ldda, npl, ldpl);
+        h->cu_busy = 0;
+        if (!st)
+          st = gemm_launch_cin(h, c.O, bf16m ? REMIFA_R_16BF : oin, REMIFA_R_32F, in1, updm1, K1,
+                               -1.0f, l1, old_, u1, old_, 1.0f, blk1, W, ldda, npu, ldpu);
+      }
     }
     if (!st && hipEventRecord(h->ev_side, h->side) != hipSuccess) st = REMIFA_ERR_HIP;
     h->stream = s0;

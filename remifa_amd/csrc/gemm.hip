@@ -499,7 +499,9 @@ int fp8_gemm_launch(remifa_handle_t h, int out_t, int64_t m, int64_t n, int64_t 
   int split = h->split_k;
   const size_t avail = h->ws_bytes > h->ws_used ? h->ws_bytes - h->ws_used : 0;
   if (split <= 0)
-    split = k > 0 ? split_model(m, n, k, TileGeom{128, 128, 2}, h->num_cu, 7.0e12, BK, avail) : 1;
+    split = k > 0 ? split_model(m, n, k, TileGeom{128, 128, 2}, std::max(1, h->num_cu - h->cu_busy),
+                                7.0e12, BK, avail)
+                  : 1;
   g.kchunk = ceil_div(ceil_div(k > 0 ? k : 1, split), BK) * BK;
   split = (int)ceil_div(k > 0 ? k : 1, g.kchunk);
   while (split > 1 && sizeof(float) * (size_t)split * m * n > avail) {
@@ -655,7 +657,7 @@ int gemm_launch_cin(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t m
     const bool standalone = h->ws_used == 0 && !h->ws_user;
     const size_t avail = standalone ? std::max<size_t>(h->ws_bytes, (size_t)256 << 20)
                                     : (h->ws_bytes > h->ws_used ? h->ws_bytes - h->ws_used : 0);
-    split = split_model(m, n, g.K, tile_geom(tile), h->num_cu,
+    split = split_model(m, n, g.K, tile_geom(tile), std::max(1, h->num_cu - h->cu_busy),
                         opk == OPK_F32 ? 0.45e12 : 3.5e12, bk, avail);
   }
   if (g.K == 0) split = 1;
