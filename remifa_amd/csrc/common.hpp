@@ -244,6 +244,13 @@ int gemm_pair_launch(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t 
                      void* c0, int64_t ldc0, int64_t m1, int64_t n1, const void* a1, int64_t lda1,
                      const void* b1, int64_t ldb1, void* c1, int64_t ldc1);
 
+// C0 -= A0 B0 (m0 x n0) and C1 -= A1 B1 (m1 x n1), fp32 C, inner size k:
+// one launch when both take unsplit 128 x 128 tiles, else two gemm_launch calls
+int gemm_dual_launch(remifa_handle_t h, int ctype, int in_t, int64_t m0, int64_t n0, int64_t k,
+                     const void* a0, int64_t lda0, const void* b0, int64_t ldb0, void* c0,
+                     int64_t ldc0, int64_t m1, int64_t n1, const void* a1, int64_t lda1,
+                     const void* b1, int64_t ldb1, void* c1, int64_t ldc1);
+
 // FP8 operands: per-32-block scaled e4m3 codes of the rows (rows = true:
 // element (r, k) at src[r + k * ld]) or columns (element (k, r) at
 // src[k + r * ld]) of a column-major fp16 / fp32 matrix; codes q[r * ldq + k],

@@ -541,10 +541,8 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
               } else {
                 const void* lr = at(obase, otyp, ofs, op, old_);
                 const void* ur = at(obase, otyp, op, ofs, old_);
-                RF_CHECK(gemm_launch(h, c.O, oin, REMIFA_R_32F, updm, in, nb, -1.0f, lr, old_,
-                                     ur, old_, 1.0f, cpl, ldpl));
-                RF_CHECK(gemm_launch(h, c.O, oin, REMIFA_R_32F, in, updm, nb, -1.0f, lr, old_,
-                                     ur, old_, 1.0f, cpu, ldpu));
+                RF_CHECK(gemm_dual_launch(h, c.O, oin, updm, in, nb, lr, old_, ur, old_, cpl, ldpl,
+                                          in, updm, lr, old_, ur, old_, cpu, ldpu));
               }
             }
             // the second stream's next update starts after this block's

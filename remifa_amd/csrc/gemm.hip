@@ -322,6 +322,45 @@ int launch_big(remifa_handle_t h, GemmArgs g, int split) {
   return 0;
 }
 
+constexpr int kSplitGranule = 64;  // split-K granularity: a multiple of every slab depth
+
+// The tile and split-K count of an update GEMM (operand kind opk, K >= 0),
+// before the workspace limits the split: the autotuned rule for the shape
+// (first match), else the heuristics; the caller's split, the rule's, else
+// the cost model's on the CUs not held by concurrent work.
+void plan_gemm(remifa_handle_t h, int opk, int64_t m, int64_t n, int64_t K, const void* a,
+               int64_t lda, const void* b, int64_t ldb, int& tile, int& split) {
+  int rule_tile = -1, rule_split = 0;
+  for (const auto& r : h->gemm_rules) {
+    if ((r[0] < 0 || r[0] == opk) && m >= r[1] && m <= r[2] && n >= r[3] && n <= r[4]) {
+      rule_tile = (int)r[5];
+      rule_split = (int)r[6];
+      break;
+    }
+  }
+  tile = choose_tile(opk, m, n);
+  auto fits = [&](int t) {
+    return t >= 0 && (opk != OPK_F32 || t == T256x32 || t == T32x256 || t == T128x128);
+  };
+  if (fits(rule_tile)) tile = rule_tile;  // autotuned table
+  // below K = 2048 the 256 x 256 kernel's start-up and epilogue outweigh its
+  // loop (measured on MI355X at N = 512): 128 x 128 unless forced
+  if (tile == T256x256 && K < 2048) tile = T128x128;
+  if (fits(h->gemm_tile)) tile = h->gemm_tile;  // explicit override (autotuner runs)
+  if (tile == T256x256 && !big_fits(opk, m, n, K, a, lda, b, ldb)) tile = T128x128;
+  split = h->split_k > 0 ? h->split_k : rule_split;
+  if (split <= 0) {
+    const bool standalone = h->ws_used == 0 && !h->ws_user;
+    const size_t avail = standalone ? std::max<size_t>(h->ws_bytes, (size_t)256 << 20)
+                                    : (h->ws_bytes > h->ws_used ? h->ws_bytes - h->ws_used : 0);
+    split = split_model(m, n, K, tile_geom(tile), std::max(1, h->num_cu - h->cu_busy),
+                        opk == OPK_F32 ? 0.45e12 : 3.5e12, kSplitGranule, avail);
+  }
+  if (K == 0) split = 1;
+  const int64_t kc = ceil_div(ceil_div(K > 0 ? K : 1, split), kSplitGranule) * kSplitGranule;
+  split = (int)ceil_div(K > 0 ? K : 1, kc);
+}
+
 template <int OPK, typename OutT>
 int dispatch_basic(remifa_handle_t h, int tile, GemmArgs g, int split, bool vec) {
   switch (tile) {
@@ -368,7 +407,72 @@ int launch_pair(remifa_handle_t h, GemmArgs g0, GemmArgs g1, bool vec) {
   return 0;
 }
 
+template <int OPK, typename OutT>
+int launch_dual(remifa_handle_t h, GemmArgs g0, GemmArgs g1, bool vec) {
+  constexpr int BK = slab_k(OPK, 128, 128);
+  g0.tiles_m = (int)ceil_div(g0.M, 128);
+  g0.tiles_n = (int)ceil_div(g0.N, 128);
+  g1.tiles_m = (int)ceil_div(g1.M, 128);
+  g1.tiles_n = (int)ceil_div(g1.N, 128);
+  const int nt0 = g0.tiles_m * g0.tiles_n;
+  dim3 grid((unsigned)(nt0 + g1.tiles_m * g1.tiles_n));
+  if (vec)
+    gemm_dual_kernel<OPK, OutT, BK, true><<<grid, 256, 0, h->stream>>>(g0, g1, nt0);
+  else
+    gemm_dual_kernel<OPK, OutT, BK, false><<<grid, 256, 0, h->stream>>>(g0, g1, nt0);
+  RF_HIP(hipGetLastError());
+  return 0;
+}
+
 }  // namespace
+
+// C0 -= A0 B0 (m0 x n0) and C1 -= A1 B1 (m1 x n1), fp32 C, inner size k: one
+// launch when both would take an unsplit 128 x 128 launch with the same
+// operand access (fp16 / bf16 operands, TC32 / BF16_TC32), else two
+// gemm_launch calls. Same tiles, hence the same bits, either way.
+int gemm_dual_launch(remifa_handle_t h, int ctype, int in_t, int64_t m0, int64_t n0, int64_t k,
+                     const void* a0, int64_t lda0, const void* b0, int64_t ldb0, void* c0,
+                     int64_t ldc0, int64_t m1, int64_t n1, const void* a1, int64_t lda1,
+                     const void* b1, int64_t ldb1, void* c1, int64_t ldc1) {
+  int opk = -1;
+  if (ctype == REMIFA_TC32 && in_t == REMIFA_R_16F) opk = OPK_F16;
+  if (ctype == REMIFA_BF16_TC32 && in_t == REMIFA_R_16F) opk = OPK_BF16;
+  auto vec_ok = [&](const void* a, int64_t lda, const void* b, int64_t ldb, int64_t m) {
+    return ((uintptr_t)a % 16 == 0) && ((uintptr_t)b % 16 == 0) && (lda % 8 == 0) &&
+           (ldb % 8 == 0) && (m % 8 == 0) && (k % 8 == 0);
+  };
+  bool dual = opk >= 0 && k > 0 && m0 > 0 && n0 > 0 && m1 > 0 && n1 > 0;
+  bool v0 = false, v1 = false;
+  if (dual) {
+    int t0, s0, t1, s1;
+    plan_gemm(h, opk, m0, n0, k, a0, lda0, b0, ldb0, t0, s0);
+    plan_gemm(h, opk, m1, n1, k, a1, lda1, b1, ldb1, t1, s1);
+    v0 = vec_ok(a0, lda0, b0, ldb0, m0);
+    v1 = vec_ok(a1, lda1, b1, ldb1, m1);
+    dual = t0 == T128x128 && t1 == T128x128 && s0 == 1 && s1 == 1 && v0 == v1;
+  }
+  if (!dual) {
+    RF_CHECK(gemm_launch(h, ctype, in_t, REMIFA_R_32F, m0, n0, k, -1.0f, a0, lda0, b0, ldb0, 1.0f,
+                         c0, ldc0));
+    return gemm_launch(h, ctype, in_t, REMIFA_R_32F, m1, n1, k, -1.0f, a1, lda1, b1, ldb1, 1.0f,
+                       c1, ldc1);
+  }
+  auto args = [&](int64_t m, int64_t n, const void* a, int64_t lda, const void* b, int64_t ldb,
+                  void* c, int64_t ldc) {
+    GemmArgs g{};
+    g.M = m, g.N = n, g.K = k, g.lda = lda, g.ldb = ldb, g.ldc = ldc;
+    g.alpha = -1.0f, g.beta = 1.0f, g.A = a, g.B = b, g.C = c;
+    g.kchunk = ceil_div(k, kSplitGranule) * kSplitGranule;  // one K slice
+    return g;
+  };
+  const GemmArgs g0 = args(m0, n0, a0, lda0, b0, ldb0, c0, ldc0);
+  const GemmArgs g1 = args(m1, n1, a1, lda1, b1, ldb1, c1, ldc1);
+  timer_begin(h, KC_GEMM, 2.0 * (double)k * (m0 * n0 + m1 * n1), (opk << 8) | T128x128);
+  const int st = opk == OPK_F16 ? launch_dual<OPK_F16, float>(h, g0, g1, v0)
+                                : launch_dual<OPK_BF16, float>(h, g0, g1, v0);
+  timer_end(h);
+  return st;
+}
 
 // The two slice updates of the inner LL factorization in one launch when
 // they take the narrow tiles without split-K (C0 -= A0 B0 with N0 <= 32 per
@@ -629,38 +733,9 @@ int gemm_launch_cin(remifa_handle_t h, int ctype, int in_t, int out_t, int64_t m
   // whole 16-byte chunks: aligned operands, M and K multiples of the chunk
   const bool vec = ((uintptr_t)a % 16 == 0) && ((uintptr_t)b % 16 == 0) && (lda % chunk == 0) &&
                    (ldb % chunk == 0) && (m % chunk == 0) && (g.K % chunk == 0);
-  const int bk = 64;  // split granularity: a multiple of every slab depth
-
-  // autotuned rule for this shape (first match), else the heuristics
-  int rule_tile = -1, rule_split = 0;
-  for (const auto& r : h->gemm_rules) {
-    if ((r[0] < 0 || r[0] == opk) && m >= r[1] && m <= r[2] && n >= r[3] && n <= r[4]) {
-      rule_tile = (int)r[5];
-      rule_split = (int)r[6];
-      break;
-    }
-  }
-  int tile = choose_tile(opk, m, n);
-  auto fits = [&](int t) {
-    return t >= 0 && (opk != OPK_F32 || t == T256x32 || t == T32x256 || t == T128x128);
-  };
-  if (fits(rule_tile)) tile = rule_tile;  // autotuned table
-  // below K = 2048 the 256 x 256 kernel's start-up and epilogue outweigh its
-  // loop (measured on MI355X at N = 512): 128 x 128 unless forced
-  if (tile == T256x256 && g.K < 2048) tile = T128x128;
-  if (fits(h->gemm_tile)) tile = h->gemm_tile;  // explicit override (autotuner runs)
-  if (tile == T256x256 && !big_fits(opk, m, n, g.K, a, lda, b, ldb)) tile = T128x128;
-
-  // split-K: the caller's, the rule's, else the cost model's
-  int split = h->split_k > 0 ? h->split_k : rule_split;
-  if (split <= 0) {
-    const bool standalone = h->ws_used == 0 && !h->ws_user;
-    const size_t avail = standalone ? std::max<size_t>(h->ws_bytes, (size_t)256 << 20)
-                                    : (h->ws_bytes > h->ws_used ? h->ws_bytes - h->ws_used : 0);
-    split = split_model(m, n, g.K, tile_geom(tile), std::max(1, h->num_cu - h->cu_busy),
-                        opk == OPK_F32 ? 0.45e12 : 3.5e12, bk, avail);
-  }
-  if (g.K == 0) split = 1;
+  const int bk = kSplitGranule;
+  int tile, split;
+  plan_gemm(h, opk, m, n, g.K, a, lda, b, ldb, tile, split);
   g.kchunk = ceil_div(ceil_div(g.K > 0 ? g.K : 1, split), bk) * bk;
   split = (int)ceil_div(g.K > 0 ? g.K : 1, g.kchunk);
   if (split > 1) {

@@ -470,3 +470,21 @@ __global__ void __launch_bounds__(256) gemm_pair_kernel(GemmArgs g0, GemmArgs g1
     gemm_mfma_body<OPK, OutT, 32, 256, 1, 4, BK, VEC, false>(g1, b - nt0, 0, false, smem,
                                                              smem + 2 * G1::A_ELEMS);
 }
+
+// Two independent GEMMs on 128 x 128 tiles in one launch (no split-K):
+// workgroups [0, nt0) run g0, the rest g1. The LL driver's K = nb parts of
+// an outer block's update (the block column, M x nb, and the block row,
+// nb x M) run on the handle's stream while the second stream waits for
+// them; one launch drops the gap between two.
+template <int OPK, typename OutT, int BK, bool VEC>
+__global__ void __launch_bounds__(256) gemm_dual_kernel(GemmArgs g0, GemmArgs g1, int nt0) {
+  using L = typename OpTraits<OPK>::Lds;
+  using GEO = TileLdsGeom<OPK, 128, 128, BK>;
+  __shared__ __attribute__((aligned(16))) L As[2 * GEO::A_ELEMS];
+  __shared__ __attribute__((aligned(16))) L Bs[2 * GEO::B_ELEMS];
+  const int b = blockIdx.x;
+  if (b < nt0)
+    gemm_mfma_body<OPK, OutT, 128, 128, 2, 2, BK, VEC, false>(g0, b, 0, false, As, Bs);
+  else
+    gemm_mfma_body<OPK, OutT, 128, 128, 2, 2, BK, VEC, false>(g1, b - nt0, 0, false, As, Bs);
+}
