@@ -791,10 +791,12 @@ int blockll_helpers(remifa_handle_t h, int64_t updm, int64_t in) {
 }
 
 // CUs the block kernels hold for an outer block while they run: the
-// one-launch form's workgroups (161 KB of LDS each, one per CU); 0 for the
-// two-launch form (one workgroup, then a short far kernel)
+// one-launch form's workgroups (161 KB of LDS each, one per CU). The
+// two-launch form is given the same figure (its far kernel holds about as
+// many for part of the time), so that both plan the same update GEMMs and
+// give the same bits.
 int blockll_cus(remifa_handle_t h, int64_t updm, int64_t in) {
-  return h->blk_fused ? 1 + blockll_helpers(h, updm, in) : 0;
+  return 1 + blockll_helpers(h, updm, in);
 }
 
 // Outer block kk: pl / pu the fp32 buffers, a the fp16 factors at the
