@@ -94,11 +94,16 @@ int remifa_set_tuning(remifa_handle_t handle, int panel_width, int split_k);
  * TC32, ib = 32): one_launch = 1 (the default) runs the diagonal tiles and
  * the rows / columns below / right of them in one launch, the latter slice
  * by slice as the diagonal role publishes each slice; 0 runs them as two
- * launches (the diagonal block, then the rest), one after the other. units_per_wave (0 = automatic, 3): units per helper
- * wavefront of the one-launch form, which sets its grid. Results are the
- * same bit for bit either way. No reference counterpart (the reference's
- * ib loop is sequential launches, src/lu_nopiv_inner_ll.hxx:197-370). */
-int remifa_set_block_kernels(remifa_handle_t handle, int one_launch, int units_per_wave);
+ * launches (the diagonal block, then the rest), one after the other.
+ * units_per_wave (0 = automatic, 3): units per helper wavefront of the
+ * one-launch form, which sets its grid. wait_polls (0 = automatic, 2^21):
+ * polls of a published counter before a wait of the one-launch form gives
+ * up; a wait that gives up makes remifa_lu_nopiv return REMIFA_ERR_HIP (a
+ * tiny value forces that path in tests). Results are the same bit for bit
+ * either way. No reference counterpart (the reference's ib loop is
+ * sequential launches, src/lu_nopiv_inner_ll.hxx:197-370). */
+int remifa_set_block_kernels(remifa_handle_t handle, int one_launch, int units_per_wave,
+                             int64_t wait_polls);
 /* Force the output tile of the update GEMMs (-1 = chosen per shape):
  * 0: 256x32, 1: 32x256, 2: 256x64, 3: 64x256, 4: 256x256, 5: 128x128,
  * 6: 256x128, 7: 128x256 (fp32 operands only take 0, 1, 5). Tile 4 runs

@@ -48,7 +48,7 @@ SIGNATURES = {
     "remifa_set_workspace": [_vp, _vp, _sz],
     "remifa_lu_workspace_size": [_vp, _i, C.POINTER(LuConfig), _i64, _i64, _i64, C.POINTER(_sz)],
     "remifa_set_tuning": [_vp, _i, _i],
-    "remifa_set_block_kernels": [_vp, _i, _i],
+    "remifa_set_block_kernels": [_vp, _i, _i, _i64],
     "remifa_set_gemm_tile": [_vp, _i],
     "remifa_set_lookahead": [_vp, _i, _i],
     "remifa_get_lookahead": [_vp, _pi, _pi],

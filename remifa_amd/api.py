@@ -114,10 +114,12 @@ class Handle:
     def set_tuning(self, panel_width: int = 0, split_k: int = 0):
         check(lib().remifa_set_tuning(self._h, panel_width, split_k), "remifa_set_tuning")
 
-    def set_block_kernels(self, one_launch: bool = True, units_per_wave: int = 0):
-        """LL block kernels: one launch (default) or diagonal then far kernel"""
-        check(lib().remifa_set_block_kernels(self._h, int(one_launch), units_per_wave),
-              "remifa_set_block_kernels")
+    def set_block_kernels(self, one_launch: bool = True, units_per_wave: int = 0,
+                          wait_polls: int = 0):
+        """LL block kernels: one launch (default) or diagonal then far kernel;
+        units per helper wavefront and polls per wait of the former (0 = auto)"""
+        check(lib().remifa_set_block_kernels(self._h, int(one_launch), units_per_wave,
+                                             wait_polls), "remifa_set_block_kernels")
 
     def set_gemm_tile(self, tile: int = -1):
         check(lib().remifa_set_gemm_tile(self._h, tile), "remifa_set_gemm_tile")

@@ -112,6 +112,7 @@ struct BlkArgs {
   int* sync;
   int tbase;
   int epoch;
+  long long wait_limit;  // polls of a published counter per wait
 };
 
 // LDS of the diagonal kernel (one workgroup; 162,048 bytes of the CU's 160 KiB)
@@ -488,7 +489,7 @@ __device__ __forceinline__ void stage_panel(_Float16 (*pan)[PP], const half_t* s
   }
 }
 
-constexpr long long kWaitLimit = 1LL << 21;  // polls of a published counter per wait
+constexpr long long kWaitLimit = 1LL << 21;  // default polls of a published counter per wait
 
 template <int MODE, bool PUB>
 __device__ void diagonal_role(const BlkArgs& g, DiagLds& S) {
@@ -606,7 +607,7 @@ __device__ void diagonal_role(const BlkArgs& g, DiagLds& S) {
             for (long long it = 0;; ++it) {
               if (__hip_atomic_load(prog, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= want)
                 break;
-              if (it > kWaitLimit ||
+              if (it > g.wait_limit ||
                   ((it & 255) == 0 &&
                    __hip_atomic_load(g.sync + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
                 __hip_atomic_store(g.sync + 2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -685,7 +686,7 @@ __device__ void helper_role(const BlkArgs& g, DiagLds& S, int hx, int nhelp) {
       for (long long it = 0;; ++it) {
         if (__hip_atomic_load(g.sync + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= want)
           break;
-        if (it > kWaitLimit ||
+        if (it > g.wait_limit ||
             ((it & 255) == 255 &&
              __hip_atomic_load(g.sync + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
           // give up (the host reports the error and the results are void)
@@ -827,6 +828,7 @@ int blockll_launch(remifa_handle_t h, void* base, int64_t m, int64_t nb, int64_t
   g.nfw = (g.nfu + BLT / 64 - 1) / (BLT / 64);
   const double bytes = 2.0 * (double)updm * in * (4 + 2 + 2);
   g.sync = blockll_sync_words(base, m, nb);
+  g.wait_limit = h->blk_wait_polls > 0 ? h->blk_wait_polls : kWaitLimit;
   if (h->blk_fused) {
     const int nhelp = blockll_helpers(h, updm, in);
     if (seq[0] == 0) blockll_sync_reset_kernel<<<1, 64, 0, h->stream>>>(g.sync);

@@ -79,6 +79,7 @@ struct remifa_handle_st {
   // helper wavefront of the one-launch form (0 = automatic)
   int blk_fused = 1;
   int blk_upw = 0;
+  long long blk_wait_polls = 0;  // polls per wait of the one-launch form (0 = 2^21)
   // CUs that concurrent work of the same driver holds while the next update
   // GEMMs run (the split-K model plans their rounds on the rest)
   int cu_busy = 0;

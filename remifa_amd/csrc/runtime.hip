@@ -142,12 +142,15 @@ int remifa_set_tuning(remifa_handle_t h, int panel_width, int split_k) {
   return 0;
 }
 
-int remifa_set_block_kernels(remifa_handle_t h, int one_launch, int units_per_wave) {
+int remifa_set_block_kernels(remifa_handle_t h, int one_launch, int units_per_wave,
+                             int64_t wait_polls) {
   if (!h) return -1;
   if (one_launch != 0 && one_launch != 1) return -2;
   if (units_per_wave < 0 || units_per_wave > 64) return -3;
+  if (wait_polls < 0) return -4;
   h->blk_fused = one_launch;
   h->blk_upw = units_per_wave;
+  h->blk_wait_polls = wait_polls;
   return 0;
 }
 

@@ -375,6 +375,37 @@ def test_block_kernels_one_launch_bit_identical(handle, gpu, n, nb, ld_pad, upw,
         _check(out[True], o, f64, orc.F16, f"one-launch block kernels n={n} nb={nb}")
 
 
+def test_block_kernels_wait_limit_reports_error(handle, gpu):
+    """Failure handling of the one-launch block kernels, forced from the
+    host: with one poll per wait the helpers give up waiting for the
+    diagonal role's first slice (every kernel still runs to its end, all
+    accesses in bounds), remifa_lu_nopiv reports REMIFA_ERR_HIP with its
+    message, and the next factorization with the default limit runs normally
+    and gives the two-launch form's bits."""
+    import torch
+    rf = _rf()
+    n, nb = 2048, 512
+    a32 = orc.gen_diagdom_alpha(n, 17, 1.1)
+    a16 = orc.f32_to_f16_bits(a32)
+    cfg = rf.api.make_config(W=rf.R_16F, I=rf.TC32, O=rf.TC32, OP=rf.R_32F, ib=32)
+    try:
+        handle.set_block_kernels(True, 0, 1)
+        da = to_dev(a16, gpu)
+        with pytest.raises(rf.RemifaError) as ei:
+            rf.lu_nopiv(handle, da, nb, algo=rf.LL, cfg=cfg)
+        assert ei.value.status == 1000 and "timed out" in str(ei.value), str(ei.value)
+        out = {}
+        for one in (True, False):
+            handle.set_block_kernels(one, 0, 0)
+            da = to_dev(a16, gpu)
+            assert rf.lu_nopiv(handle, da, nb, algo=rf.LL, cfg=cfg) == 0
+            torch.cuda.synchronize()
+            out[one] = to_np(da)
+    finally:
+        handle.set_block_kernels(True, 0, 0)
+    assert out[True].tobytes() == out[False].tobytes()
+
+
 @pytest.mark.parametrize("n,nb,ld_pad,lookahead", [(1536, 512, 0, True), (2048, 256, 8, True),
                                                    (1536, 512, 0, False), (1280, 256, 0, True)])
 def test_large_tile_updates_match_oracle(handle, gpu, n, nb, ld_pad, lookahead):
