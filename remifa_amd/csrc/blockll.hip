@@ -491,6 +491,37 @@ __device__ __forceinline__ void stage_panel(_Float16 (*pan)[PP], const half_t* s
 
 constexpr long long kWaitLimit = 1LL << 21;  // default polls of a published counter per wait
 
+// stage_panel for both of the diagonal role's panels with every load of a
+// thread issued before its first LDS store (so <= NBM - SW: at most
+// PANEL_CHUNKS 16-byte chunks per thread and panel), one memory round trip
+// on the tile chain instead of one per chunk
+constexpr int PANEL_CHUNKS = (SW * (NBM - SW) / 8 + BLT - 1) / BLT;
+__device__ __forceinline__ void stage_panels(_Float16 (*pan0)[PP], const half_t* src0,
+                                             _Float16 (*pan1)[PP], const half_t* src1,
+                                             int64_t ldo, int so) {
+  const int kc = so / 8;  // 16-byte chunks per row
+  const int tot = SW * kc;
+  bh8 v0[PANEL_CHUNKS], v1[PANEL_CHUNKS];
+#pragma unroll
+  for (int u = 0; u < PANEL_CHUNKS; ++u) {
+    const int c = threadIdx.x + u * BLT;
+    if (c < tot) {
+      const int e = c / kc, k = 8 * (c % kc);
+      v0[u] = *(const bh8*)(src0 + (int64_t)(so + e) * ldo + k);
+      v1[u] = *(const bh8*)(src1 + (int64_t)(so + e) * ldo + k);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < PANEL_CHUNKS; ++u) {
+    const int c = threadIdx.x + u * BLT;
+    if (c < tot) {
+      const int e = c / kc, k = 8 * (c % kc);
+      *(bh8*)&pan0[e][k] = v0[u];
+      *(bh8*)&pan1[e][k] = v1[u];
+    }
+  }
+}
+
 template <int MODE, bool PUB>
 __device__ void diagonal_role(const BlkArgs& g, DiagLds& S) {
   using Sm = Sem<MODE>;
@@ -504,8 +535,7 @@ __device__ void diagonal_role(const BlkArgs& g, DiagLds& S) {
     // 0. the slice's operands: op(U) columns of the tile (the rows of L
     // take them) and op(L) rows of the tile (the columns of U take them),
     // k < so
-    stage_panel(S.pan[0], g.uop, g.ldo, so);
-    stage_panel(S.pan[1], g.lop, g.ldo, so);
+    stage_panels(S.pan[0], g.uop, S.pan[1], g.lop, g.ldo, so);
     __syncthreads();
     // 1. the tile's update T = C - op(L[so.., 0:so]) op(U[0:so, so..]):
     // wavefront w takes quarter (ti, tj) = w % 4 over half kh = w / 4 of
