@@ -575,18 +575,14 @@ __device__ void diagonal_role(const BlkArgs& g, DiagLds& S) {
     tile_factor<MODE, false, 32>(S.f.fac, x, 32, w, i);
     __syncthreads();
     const float rk = S.f.fac.R[i];
-    // 3. the factored tile: fp16 factors, the slice table (LDS and global)
+    // 3. the factored tile: fp16 factors, the slice table (LDS, then copied
+    // to global memory with 16-byte stores: every entry is written below)
     SliceTab* gt = g.tabs + s;
 #pragma unroll
     for (int jj = 0; jj < TP; ++jj) g.a[(so + i) + (int64_t)(so + cb + jj) * g.lda] = (half_t)x[jj];
-    auto put = [&](int role, int b, int k, int e, float v) {
-      S.tab.role[role].m[b][k][e] = v;
-      gt->role[role].m[b][k][e] = v;
-    };
+    auto put = [&](int role, int b, int k, int e, float v) { S.tab.role[role].m[b][k][e] = v; };
     auto put_o = [&](int role, int e, int k, float v) {
-      const _Float16 o = (_Float16)Sm::op(v);
-      S.tab.role[role].mot[e][k] = o;
-      gt->role[role].mot[e][k] = o;
+      S.tab.role[role].mot[e][k] = (_Float16)Sm::op(v);
     };
 #pragma unroll
     for (int jj = 0; jj < TP; ++jj) {
@@ -605,6 +601,13 @@ __device__ void diagonal_role(const BlkArgs& g, DiagLds& S) {
       } else {              // i >= 32, c < 32: L below block 0
         put_o(1, i - 32, c, x[jj]);
       }
+    }
+    __syncthreads();
+    {
+      constexpr int NT = (int)(sizeof(SliceTab) / 16);
+      const float4* src = (const float4*)&S.tab;
+      float4* dst = (float4*)gt;
+      for (int e = t; e < NT; e += BLT) dst[e] = src[e];
     }
     // (publish: every wavefront's stores have reached the L2 before the
     // barrier; the release store after it writes the L2 back once)
