@@ -338,15 +338,17 @@ def test_block_kernel_matches_slices_and_oracle(handle, gpu, n, nb, ld_pad):
 @pytest.mark.parametrize("n,nb,ld_pad,upw,lookahead,oracle",
                          [(1536, 512, 0, 0, True, True), (1000, 256, 4, 1, True, True),
                           (640, 192, 8, 3, False, True), (777, 128, 0, 64, True, True),
-                          (4096, 512, 0, 0, True, False), (3000, 512, 0, 5, True, False)])
+                          (4096, 512, 0, 0, True, False), (3000, 512, 0, 5, True, False),
+                          (640, 64, 0, 0, True, True), (512, 512, 0, 0, False, True)])
 def test_block_kernels_one_launch_bit_identical(handle, gpu, n, nb, ld_pad, upw, lookahead,
                                                 oracle):
     """The block kernels' one-launch form (the diagonal role publishing each
     slice, the far units following it in the same grid) against the
     two-launch form: bit-identical factors, with the helper count varied
     (far units per wavefront 1 .. 64; 0 = automatic), ragged blocks and
-    units cut by the matrix edge; against the oracle where it finishes in
-    seconds."""
+    units cut by the matrix edge, one slice per block (nb = 64: no
+    diagonal-block units for the helpers) and a single outer block (no far
+    units); against the oracle where it finishes in seconds."""
     import torch
     rf = _rf()
     a32 = orc.gen_diagdom_alpha(n, 13 + n, 1.1)
