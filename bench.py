@@ -880,8 +880,8 @@ def bench_headline(a):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU "
-              "design is described in DESIGN.md and not built.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); sharding the "
+              "factorization over several GPUs is not built.", file=sys.stderr)
         sys.exit(2)
     if a.workload == "batched":
         print(json.dumps(bench_batched(a)), flush=True)
