@@ -9,7 +9,10 @@
 // ticket, so every block a workgroup waits for belongs to a workgroup that is
 // already running (no deadlock whatever the residency). Block row i
 // accumulates B_i - sum_j T_ij X_j, multiplies by its inverted diagonal block
-// and publishes X_i as tagged 64-bit words (value, 1) written with
+// (with one right-hand side the last dependency is folded into it ahead of
+// time: X_i = (T_ii)^-1 (B_i - sum of the others) - ((T_ii)^-1 T_i,i-1)
+// X_i-1, so one product and one reduction follow the last wait instead of
+// two of each) and publishes X_i as tagged 64-bit words (value, 1) written with
 // agent-scope atomic stores into a tag buffer cleared before the launch: a
 // reader retries until every word it needs carries the tag, so a value and
 // its "published" mark arrive together (one memory round trip per hop
@@ -183,34 +186,55 @@ __global__ void __launch_bounds__(TT) trsm_kernel(int64_t n, int nrhs, const LT*
 #pragma unroll
   for (int k = 0; k < NR; ++k) acc[k] = 0.0f;
 
-  const int nsteps = tk;  // blocks before this one in dependency order
+  // One right-hand side: the last dependency (block tk-1 in dependency
+  // order, the "near" block) is folded into the diagonal block's inverse
+  // ahead of time, M = (T_ii)^-1 T_i,near, so that once X_near is published
+  // only X_i = C - M X_near remains (one product and one reduction), with
+  // C = (T_ii)^-1 (B_i - sum over the other blocks) formed while waiting.
+  constexpr bool FOLD = NR == 1;
+  const bool fold = FOLD && tk > 0;
+  const int nsteps = fold ? tk - 1 : tk;  // blocks summed before the wait for the near one
   const int64_t grc = gr < n ? gr : n - 1;  // in-bounds row for the loads
   // one right-hand side: this lane's entry of b (lanes cq = 0), loaded now
   // so that no memory round trip follows the last dependency wait (only
   // this workgroup writes these rows of b). With 16 right-hand sides the
   // extra registers would spill; those are loaded after the waits.
   const float bpf = NR == 1 ? b[grc] : 0.0f;
-  for (int s0 = 0; s0 < nsteps; s0 += CH) {
-    const int cnt = nsteps - s0 < CH ? nsteps - s0 : CH;
-    // this lane's entries of the chunk's blocks of T: issued before the wait
-    // (they do not depend on it), with clamped addresses and no branches so
-    // the loads are all in flight together
-    float tv[CH][QC];
+  float mv[FOLD ? QC : 1];
+  // one right-hand side: (T_ii)^-1 lives in LDS from here on (rows padded
+  // against bank conflicts), which frees the lane's registers for the loop
+  __shared__ float Ds[FOLD ? NBS : 1][NBS + 1];
+  if constexpr (FOLD) {
 #pragma unroll
-    for (int q = 0; q < CH; ++q) {
-      const int qq = q < cnt ? q : cnt - 1;
-      const int bj = LOWER ? s0 + qq : nblk - 1 - (s0 + qq);
-      const int64_t c0 = (int64_t)bj * NBS + cq * QC;
+    for (int c = 0; c < QC; ++c) Ds[row][cq * QC + c] = dv[c];
+    if (fold) {
+      // M: this lane's row and column quarter, from (T_ii)^-1 and T_i,near
+      // in LDS
+      __shared__ float Ts[NBS][NBS];
+      const int bjn = LOWER ? tk - 1 : nblk - 1 - (tk - 1);
+      const int64_t c0 = (int64_t)bjn * NBS + cq * QC;
 #pragma unroll
       for (int c = 0; c < QC; ++c) {
         const int64_t C = c0 + c < n ? c0 + c : n - 1;
         const float v = tel<LT, TRANS>(lu, ld, grc, C);
-        tv[q][c] = (gr < n && c0 + c < n) ? v : 0.0f;
+        Ts[row][cq * QC + c] = (gr < n && c0 + c < n) ? v : 0.0f;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < QC; ++c) mv[c] = 0.0f;
+#pragma unroll 2
+      for (int k = 0; k < NBS; ++k) {
+        const float d = Ds[row][k];
+#pragma unroll
+        for (int c = 0; c < QC; ++c) mv[c] = __builtin_fmaf(d, Ts[k][cq * QC + c], mv[c]);
       }
     }
-    // stage the chunk's solution blocks: agent-scope atomic loads of the
-    // tagged words, retried until every one carries the published tag
-    // (uniform decisions: a barrier vote and a double-buffered stop flag)
+  }
+  // the solution blocks s0 .. s0 + cnt - 1 (dependency order) into Xs:
+  // agent-scope atomic loads of the tagged words, retried until every one
+  // carries the published tag (uniform decisions: a barrier vote and a
+  // double-buffered stop flag)
+  auto stage_blocks = [&](int s0, int cnt) {
     for (long long tries = 0;; ++tries) {
       int bad = 0;
       for (int e = t; e < cnt * NBS * NR; e += TT) {
@@ -240,6 +264,26 @@ __global__ void __launch_bounds__(TT) trsm_kernel(int64_t n, int nrhs, const LT*
       }
       __builtin_amdgcn_s_sleep(2);
     }
+  };
+  for (int s0 = 0; s0 < nsteps; s0 += CH) {
+    const int cnt = nsteps - s0 < CH ? nsteps - s0 : CH;
+    // this lane's entries of the chunk's blocks of T: issued before the wait
+    // (they do not depend on it), with clamped addresses and no branches so
+    // the loads are all in flight together
+    float tv[CH][QC];
+#pragma unroll
+    for (int q = 0; q < CH; ++q) {
+      const int qq = q < cnt ? q : cnt - 1;
+      const int bj = LOWER ? s0 + qq : nblk - 1 - (s0 + qq);
+      const int64_t c0 = (int64_t)bj * NBS + cq * QC;
+#pragma unroll
+      for (int c = 0; c < QC; ++c) {
+        const int64_t C = c0 + c < n ? c0 + c : n - 1;
+        const float v = tel<LT, TRANS>(lu, ld, grc, C);
+        tv[q][c] = (gr < n && c0 + c < n) ? v : 0.0f;
+      }
+    }
+    stage_blocks(s0, cnt);
 #pragma unroll
     for (int q = 0; q < CH; ++q) {
       if (q < cnt) {
@@ -254,7 +298,8 @@ __global__ void __launch_bounds__(TT) trsm_kernel(int64_t n, int nrhs, const LT*
     __syncthreads();
   }
 
-  // right-hand side of the diagonal block: B_i - sum_j T_ij X_j
+  // right-hand side of the diagonal block: B_i - sum_j T_ij X_j (over the
+  // blocks summed so far)
 #pragma unroll
   for (int k = 0; k < NR; ++k) Red[cq][row][k] = acc[k];
   __syncthreads();
@@ -264,18 +309,40 @@ __global__ void __launch_bounds__(TT) trsm_kernel(int64_t n, int nrhs, const LT*
     Xs[row][k] = (gr < n && k < nrhs) ? bv - sum : 0.0f;
   }
   __syncthreads();
-  // X_i = (T_ii)^-1 (B_i - ...)
+  // X_i = (T_ii)^-1 (B_i - ...)  (with the fold: C)
   float y[NR];
 #pragma unroll
   for (int k = 0; k < NR; ++k) y[k] = 0.0f;
 #pragma unroll
   for (int c = 0; c < QC; ++c) {
+    const float d = FOLD ? Ds[row][cq * QC + c] : dv[c];
 #pragma unroll
-    for (int k = 0; k < NR; ++k) y[k] = __builtin_fmaf(dv[c], Xs[cq * QC + c][k], y[k]);
+    for (int k = 0; k < NR; ++k) y[k] = __builtin_fmaf(d, Xs[cq * QC + c][k], y[k]);
   }
 #pragma unroll
   for (int k = 0; k < NR; ++k) Red[cq][row][k] = y[k];
   __syncthreads();
+  if constexpr (FOLD) {
+    if (fold) {
+      // C (lanes cq = 0), then the near block: X_i = C - M X_near
+      const float cv =
+          cq == 0 ? ((Red[0][row][0] + Red[1][row][0]) + Red[2][row][0]) + Red[3][row][0] : 0.0f;
+      __syncthreads();  // (Red and Xs are rewritten below)
+      stage_blocks(tk - 1, 1);
+      float z = 0.0f;
+#pragma unroll
+      for (int c = 0; c < QC; ++c) z = __builtin_fmaf(mv[c], Xs[cq * QC + c][0], z);
+      Red[cq][row][0] = z;
+      __syncthreads();
+      if (cq == 0 && gr < n) {
+        const float xv = cv - (((Red[0][row][0] + Red[1][row][0]) + Red[2][row][0]) + Red[3][row][0]);
+        b[gr] = xv;  // the caller's copy
+        __hip_atomic_store(&tags[gr], (1ull << 32) | (unsigned long long)__float_as_uint(xv),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      return;
+    }
+  }
   for (int k = cq; k < NR; k += 4) {
     const float xv = ((Red[0][row][k] + Red[1][row][k]) + Red[2][row][k]) + Red[3][row][k];
     if (gr < n && k < nrhs) {
