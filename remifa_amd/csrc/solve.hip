@@ -161,7 +161,7 @@ __global__ void __launch_bounds__(TT) trsm_kernel(int64_t n, int nrhs, const LT*
   constexpr int NP = NR == 1 ? 1 : NR + 1;
   __shared__ float Xs[CH * NBS][NR];
   __shared__ float Red[4][NBS][NP];
-  __shared__ int s_ticket, s_stop[2];
+  __shared__ int s_ticket;
   const int t = threadIdx.x;
   const int row = t % NBS, cq = t / NBS;
   if (t == 0)
@@ -231,9 +231,10 @@ __global__ void __launch_bounds__(TT) trsm_kernel(int64_t n, int nrhs, const LT*
     }
   }
   // the solution blocks s0 .. s0 + cnt - 1 (dependency order) into Xs:
-  // agent-scope atomic loads of the tagged words, retried until every one
-  // carries the published tag (uniform decisions: a barrier vote and a
-  // double-buffered stop flag)
+  // agent-scope atomic loads of the tagged words, each wavefront retrying
+  // its own until every one carries the published tag (a wavefront vote),
+  // then one barrier; bounded, and a wavefront that gives up raises the
+  // error flag that ends the others' waits
   auto stage_blocks = [&](int s0, int cnt) {
     for (long long tries = 0;; ++tries) {
       int bad = 0;
@@ -252,18 +253,18 @@ __global__ void __launch_bounds__(TT) trsm_kernel(int64_t n, int nrhs, const LT*
         }
         Xs[rr][k] = v;
       }
-      const int slot = (int)(tries & 1);
-      if (t == 0)
-        s_stop[slot] = tries > kSpinLimit ||
-                       __hip_atomic_load(&sync[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-      if (!__syncthreads_or(bad)) break;
-      if (s_stop[slot]) {
+      if (__ballot(bad) == 0) break;
+      if (tries > kSpinLimit ||
+          ((tries & 63) == 63 &&
+           __hip_atomic_load(&sync[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
         // give up (the host reports the error and discards the results)
-        if (t == 0) __hip_atomic_store(&sync[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((t & 63) == 0)
+          __hip_atomic_store(&sync[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         break;
       }
-      __builtin_amdgcn_s_sleep(2);
+      __builtin_amdgcn_s_sleep(1);
     }
+    __syncthreads();
   };
   for (int s0 = 0; s0 < nsteps; s0 += CH) {
     const int cnt = nsteps - s0 < CH ? nsteps - s0 : CH;
