@@ -292,7 +292,9 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
   const int W = c.W;
   if (m == 0) {
     if (d_info) zero_int_kernel<<<1, 256, 0, h->stream>>>(d_info, 1);
-    RF_HIP(hipStreamSynchronize(h->stream));
+    hipStreamCaptureStatus cap0 = hipStreamCaptureStatusNone;
+    RF_HIP(hipStreamIsCapturing(h->stream, &cap0));
+    if (cap0 == hipStreamCaptureStatusNone) RF_HIP(hipStreamSynchronize(h->stream));
     return 0;
   }
   const bool need_cpy = W == REMIFA_R_32F && (c.I != REMIFA_FP32 || c.O != REMIFA_FP32);
@@ -650,7 +652,12 @@ int lu_nopiv_impl(remifa_handle_t h, int algo, const remifa_lu_config_t* cfg, in
     info_kernel<<<1, 256, 0, h->stream>>>(n, d_a, ldda, W, d_info);
     RF_HIP(hipGetLastError());
   }
-  // the reference drivers synchronise before returning (:567-569)
+  // the reference drivers synchronise before returning (:567-569). Under
+  // HIP graph capture the call only records its kernels: no readback and no
+  // synchronisation (a replay's block-kernel wait flag is then not checked)
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  RF_HIP(hipStreamIsCapturing(h->stream, &cap));
+  if (cap != hipStreamCaptureStatusNone) return 0;
   int waited_out = 0;
   if (blk_seq[0] > 0)
     RF_HIP(hipMemcpyAsync(&waited_out, blockll_sync_words(blkws, m, nb) + 2, sizeof(int),

@@ -377,6 +377,57 @@ def test_block_kernels_one_launch_bit_identical(handle, gpu, n, nb, ld_pad, upw,
         _check(out[True], o, f64, orc.F16, f"one-launch block kernels n={n} nb={nb}")
 
 
+def test_headline_layout_lu_in_hip_graph(handle, gpu):
+    """The headline layout (fp16 storage, fp32 buffers, TC32, ib = 32, the
+    one-launch block kernels, split-K updates) captured in one HIP graph:
+    under capture the driver runs on the one stream and only records its
+    kernels (the block kernels' sync words are cleared by a kernel inside
+    the graph, no memset node). Replayed three times with eager work and
+    synchronisations between the replays: every replay's factors equal the
+    eager call's (one stream, no lookahead) bit for bit, and the oracle."""
+    import torch
+    rf = _rf()
+    n, nb = 2048, 512
+    a32 = orc.gen_diagdom_alpha(n, 23, 1.1)
+    a16 = orc.f32_to_f16_bits(a32)
+    cfg = rf.api.make_config(W=rf.R_16F, I=rf.TC32, O=rf.TC32, OP=rf.R_32F, ib=32)
+    info = torch.full((1,), -7, dtype=torch.int32, device=gpu)
+    try:
+        handle.set_lookahead(False, 0)
+        de = to_dev(a16, gpu)
+        assert rf.lu_nopiv(handle, de, nb, algo=rf.LL, cfg=cfg, info=info) == 0
+        torch.cuda.synchronize()
+        assert int(info.item()) == 0
+        eager = de.clone()
+        handle.set_lookahead(True, 0)  # (capture itself keeps the one stream)
+        buf = to_dev(a16, gpu)
+        a0 = buf.clone()
+        gs = torch.cuda.Stream()
+        graph = torch.cuda.CUDAGraph()
+        old = handle.stream
+        try:
+            handle.set_stream(gs)
+            with torch.cuda.graph(graph, stream=gs):
+                rf.lu_nopiv(handle, buf, nb, algo=rf.LL, cfg=cfg, info=info)
+        finally:
+            handle.set_stream(old)
+        for rep in range(3):
+            buf.copy_(a0)
+            info.fill_(-7)
+            torch.cuda.synchronize()
+            with torch.cuda.stream(gs):
+                graph.replay()
+            torch.cuda.synchronize()
+            _ = (buf[:100].float() * 2.0).sum().item()  # eager work between replays
+            assert int(info.item()) == 0, rep
+            assert torch.equal(buf.view(torch.int16), eager.view(torch.int16)), rep
+    finally:
+        handle.set_lookahead(True, 0)
+    o = orc.lu_nopiv("ll", orc.cfg(W=orc.F16, I=orc.TC32, O=orc.TC32, OP=orc.F32, ib=32), a16, nb)
+    f64 = orc.lu_f64(orc.round_to(orc.F16, a32))
+    _check(to_np(buf), o, f64, orc.F16, f"graph-replayed LU n={n} nb={nb}")
+
+
 def test_block_kernels_wait_limit_reports_error(handle, gpu):
     """Failure handling of the one-launch block kernels, forced from the
     host: with one poll per wait the helpers give up waiting for the
