@@ -194,6 +194,10 @@ __global__ void __launch_bounds__(TT) trsm_kernel(int64_t n, int nrhs, const LT*
   constexpr bool FOLD = NR == 1;
   const bool fold = FOLD && tk > 0;
   const int nsteps = fold ? tk - 1 : tk;  // blocks summed before the wait for the near one
+  // rows of the stored factors start on 16 bytes (16-byte vector reads of
+  // the transposed triangle's rows)
+  const bool vec_rows =
+      TRANS && ((uintptr_t)lu % 16 == 0) && (ld % (16 / (int64_t)sizeof(LT)) == 0);
   const int64_t grc = gr < n ? gr : n - 1;  // in-bounds row for the loads
   // one right-hand side: this lane's entry of b (lanes cq = 0), loaded now
   // so that no memory round trip follows the last dependency wait (only
@@ -277,6 +281,28 @@ __global__ void __launch_bounds__(TT) trsm_kernel(int64_t n, int nrhs, const LT*
       const int qq = q < cnt ? q : cnt - 1;
       const int bj = LOWER ? s0 + qq : nblk - 1 - (s0 + qq);
       const int64_t c0 = (int64_t)bj * NBS + cq * QC;
+      if (TRANS && vec_rows && gr < n && c0 + QC <= n) {
+        // the transposed triangle: the lane's 32 entries are contiguous in
+        // its row of the stored factors, read as 16-byte vectors
+        const LT* pr = lu + c0 + grc * ld;
+        if constexpr (sizeof(LT) == 2) {
+          typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+#pragma unroll
+          for (int u = 0; u < QC / 8; ++u) {
+            const h8 v = *(const h8*)(pr + 8 * u);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) tv[q][8 * u + j] = (float)v[j];
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < QC / 4; ++u) {
+            const float4 v = *(const float4*)(pr + 4 * u);
+            tv[q][4 * u] = v.x, tv[q][4 * u + 1] = v.y, tv[q][4 * u + 2] = v.z,
+                      tv[q][4 * u + 3] = v.w;
+          }
+        }
+        continue;
+      }
 #pragma unroll
       for (int c = 0; c < QC; ++c) {
         const int64_t C = c0 + c < n ? c0 + c : n - 1;
