@@ -158,7 +158,11 @@ int remifa_timing_records(remifa_handle_t handle, int64_t max, int *cls, int *va
  * d_a_cpy/ld_a_cpy: fp16 copy of the factors the reference requires when
  * W = fp32 and an update is TC32 (src/lu_nopiv_ll.hxx:132-135); pass NULL to
  * let the library use workspace instead. d_info (device int) receives 0, or
- * k > 0 when u_kk of column k (1-based) is zero or not finite. */
+ * k > 0 when u_kk of column k (1-based) is zero or not finite. Returns with
+ * the handle's stream synchronised, as the reference drivers do; while that
+ * stream is being captured into a HIP graph the call only records its
+ * kernels on it (one stream, no synchronisation or readback; size the
+ * workspace with one eager call first). */
 int remifa_lu_nopiv(remifa_handle_t handle, int algo, const remifa_lu_config_t *cfg, int64_t m,
                     int64_t n, int64_t nb, void *d_a, int64_t ldda, int *d_info, void *d_a_cpy,
                     int64_t ld_a_cpy);
