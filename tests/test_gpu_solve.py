@@ -47,6 +47,36 @@ def test_getrs_matches_f64(handle, gpu, n, nrhs, lu_t, trans):
     assert np.abs(x - x64).max() <= 64 * n * 2.0**-24 * scale
 
 
+@pytest.mark.parametrize("n,lu_t", [(1000, "f16"), (4096, "f16"), (1000, "f32"), (2500, "f32")])
+def test_getrs_transposed_aligned_rows(handle, gpu, n, lu_t):
+    """The transposed solves with 16-byte aligned rows of the stored factors
+    (ld = n, a multiple of 8 / 4): each lane's row segment is read as vectors,
+    the matrix edge and the last partial block as single entries. Against the
+    FP64 solve, and bit for bit against the same factors at ld = n + 2, where
+    every entry is read singly."""
+    import torch
+    rf = _rf()
+    a = orc.gen_diagdom_alpha(n, 5, 1.2)
+    f = orc.lu_nopiv("rl", orc.cfg(), a, 64)
+    if lu_t == "f16":
+        fb = orc.f32_to_f16_bits(f)
+        f = orc.f16_bits_to_f32(fb)
+    else:
+        fb = f
+    b = orc.gen_uniform(n, 1, 29)
+    xs = []
+    for pad in (0, 2):
+        dlu = to_dev(fb, gpu, ld=n + pad)
+        db = to_dev(b, gpu)
+        rf.getrs_nopiv(handle, dlu, db, trans=True)
+        torch.cuda.synchronize()
+        xs.append(to_np(db))
+    assert xs[0].tobytes() == xs[1].tobytes()
+    x64 = orc.getrs_f64(f.astype(np.float64), b.astype(np.float64), trans=True)
+    scale = np.abs(x64).max()
+    assert np.abs(xs[0].astype(np.float64) - x64).max() <= 64 * n * 2.0**-24 * scale
+
+
 def test_residual_f64(handle, gpu):
     import torch
     rf = _rf()
