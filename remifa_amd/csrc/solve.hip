@@ -9,10 +9,9 @@
 // ticket, so every block a workgroup waits for belongs to a workgroup that is
 // already running (no deadlock whatever the residency). Block row i
 // accumulates B_i - sum_j T_ij X_j, multiplies by its inverted diagonal block
-// (with one right-hand side the last dependency is folded into it ahead of
-// time: X_i = (T_ii)^-1 (B_i - sum of the others) - ((T_ii)^-1 T_i,i-1)
-// X_i-1, so one product and one reduction follow the last wait instead of
-// two of each) and publishes X_i as tagged 64-bit words (value, 1) written with
+// (the last dependency is folded into it ahead of time: X_i = (T_ii)^-1
+// (B_i - sum of the others) - ((T_ii)^-1 T_i,i-1) X_i-1, so one product and
+// one reduction follow the last wait instead of two of each) and publishes X_i as tagged 64-bit words (value, 1) written with
 // agent-scope atomic stores into a tag buffer cleared before the launch: a
 // reader retries until every word it needs carries the tag, so a value and
 // its "published" mark arrive together (one memory round trip per hop
@@ -159,8 +158,14 @@ __global__ void __launch_bounds__(TT) trsm_kernel(int64_t n, int nrhs, const LT*
                                                    int nblk) {
   constexpr int CH = NR == 1 ? 2 : 1;  // solution blocks staged in LDS at a time
   constexpr int NP = NR == 1 ? 1 : NR + 1;
-  __shared__ float Xs[CH * NBS][NR];
-  __shared__ float Red[4][NBS][NP];
+  // Xs and Red share their LDS with the near block of T, which is only read
+  // before the first solution block is staged (a barrier between)
+  constexpr int XS_N = CH * NBS * NR, RED_N = 4 * NBS * NP, TS_N = NBS * NBS;
+  constexpr int U_N = XS_N + RED_N > TS_N ? XS_N + RED_N : TS_N;
+  __shared__ __attribute__((aligned(16))) float ush[U_N];
+  float(*const Xs)[NR] = reinterpret_cast<float(*)[NR]>(ush);
+  float(*const Red)[NBS][NP] = reinterpret_cast<float(*)[NBS][NP]>(ush + XS_N);
+  float(*const Ts)[NBS] = reinterpret_cast<float(*)[NBS]>(ush);
   __shared__ int s_ticket;
   const int t = threadIdx.x;
   const int row = t % NBS, cq = t / NBS;
@@ -186,12 +191,12 @@ __global__ void __launch_bounds__(TT) trsm_kernel(int64_t n, int nrhs, const LT*
 #pragma unroll
   for (int k = 0; k < NR; ++k) acc[k] = 0.0f;
 
-  // One right-hand side: the last dependency (block tk-1 in dependency
-  // order, the "near" block) is folded into the diagonal block's inverse
-  // ahead of time, M = (T_ii)^-1 T_i,near, so that once X_near is published
-  // only X_i = C - M X_near remains (one product and one reduction), with
-  // C = (T_ii)^-1 (B_i - sum over the other blocks) formed while waiting.
-  constexpr bool FOLD = NR == 1;
+  // The last dependency (block tk-1 in dependency order, the "near" block)
+  // is folded into the diagonal block's inverse ahead of time, M =
+  // (T_ii)^-1 T_i,near, so that once X_near is published only X_i = C - M
+  // X_near remains (one product and one reduction), with C = (T_ii)^-1 (B_i
+  // - sum over the other blocks) formed while waiting.
+  constexpr bool FOLD = true;
   const bool fold = FOLD && tk > 0;
   const int nsteps = fold ? tk - 1 : tk;  // blocks summed before the wait for the near one
   // rows of the stored factors start on 16 bytes (16-byte vector reads of
@@ -204,17 +209,19 @@ __global__ void __launch_bounds__(TT) trsm_kernel(int64_t n, int nrhs, const LT*
   // this workgroup writes these rows of b). With 16 right-hand sides the
   // extra registers would spill; those are loaded after the waits.
   const float bpf = NR == 1 ? b[grc] : 0.0f;
-  float mv[FOLD ? QC : 1];
-  // one right-hand side: (T_ii)^-1 lives in LDS from here on (rows padded
-  // against bank conflicts), which frees the lane's registers for the loop
-  __shared__ float Ds[FOLD ? NBS : 1][NBS + 1];
+  float mv[QC];
+  // (T_ii)^-1 in LDS (rows padded against bank conflicts) for M; with one
+  // right-hand side it stays there for the diagonal product, which frees
+  // the lane's registers for the loop
+  __shared__ float Ds[NBS][NBS + 1];
   if constexpr (FOLD) {
+    if (NR == 1 || fold) {
 #pragma unroll
-    for (int c = 0; c < QC; ++c) Ds[row][cq * QC + c] = dv[c];
+      for (int c = 0; c < QC; ++c) Ds[row][cq * QC + c] = dv[c];
+    }
     if (fold) {
       // M: this lane's row and column quarter, from (T_ii)^-1 and T_i,near
       // in LDS
-      __shared__ float Ts[NBS][NBS];
       const int bjn = LOWER ? tk - 1 : nblk - 1 - (tk - 1);
       const int64_t c0 = (int64_t)bjn * NBS + cq * QC;
 #pragma unroll
@@ -232,6 +239,7 @@ __global__ void __launch_bounds__(TT) trsm_kernel(int64_t n, int nrhs, const LT*
 #pragma unroll
         for (int c = 0; c < QC; ++c) mv[c] = __builtin_fmaf(d, Ts[k][cq * QC + c], mv[c]);
       }
+      __syncthreads();  // (Ts is Xs and Red from here on)
     }
   }
   // the solution blocks s0 .. s0 + cnt - 1 (dependency order) into Xs:
@@ -342,7 +350,7 @@ __global__ void __launch_bounds__(TT) trsm_kernel(int64_t n, int nrhs, const LT*
   for (int k = 0; k < NR; ++k) y[k] = 0.0f;
 #pragma unroll
   for (int c = 0; c < QC; ++c) {
-    const float d = FOLD ? Ds[row][cq * QC + c] : dv[c];
+    const float d = NR == 1 ? Ds[row][cq * QC + c] : dv[c];
 #pragma unroll
     for (int k = 0; k < NR; ++k) y[k] = __builtin_fmaf(d, Xs[cq * QC + c][k], y[k]);
   }
@@ -351,21 +359,40 @@ __global__ void __launch_bounds__(TT) trsm_kernel(int64_t n, int nrhs, const LT*
   __syncthreads();
   if constexpr (FOLD) {
     if (fold) {
-      // C (lanes cq = 0), then the near block: X_i = C - M X_near
-      const float cv =
-          cq == 0 ? ((Red[0][row][0] + Red[1][row][0]) + Red[2][row][0]) + Red[3][row][0] : 0.0f;
+      // C (right-hand sides k = cq + 4 j in lane (row, cq)), then the near
+      // block: X_i = C - M X_near
+      constexpr int NJ = (NR + 3) / 4;
+      float cv[NJ];
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        const int k = cq + 4 * j;
+        cv[j] = k < NR ? ((Red[0][row][k] + Red[1][row][k]) + Red[2][row][k]) + Red[3][row][k]
+                       : 0.0f;
+      }
       __syncthreads();  // (Red and Xs are rewritten below)
       stage_blocks(tk - 1, 1);
-      float z = 0.0f;
+      float z[NR];
 #pragma unroll
-      for (int c = 0; c < QC; ++c) z = __builtin_fmaf(mv[c], Xs[cq * QC + c][0], z);
-      Red[cq][row][0] = z;
+      for (int k = 0; k < NR; ++k) z[k] = 0.0f;
+#pragma unroll
+      for (int c = 0; c < QC; ++c) {
+#pragma unroll
+        for (int k = 0; k < NR; ++k) z[k] = __builtin_fmaf(mv[c], Xs[cq * QC + c][k], z[k]);
+      }
+#pragma unroll
+      for (int k = 0; k < NR; ++k) Red[cq][row][k] = z[k];
       __syncthreads();
-      if (cq == 0 && gr < n) {
-        const float xv = cv - (((Red[0][row][0] + Red[1][row][0]) + Red[2][row][0]) + Red[3][row][0]);
-        b[gr] = xv;  // the caller's copy
-        __hip_atomic_store(&tags[gr], (1ull << 32) | (unsigned long long)__float_as_uint(xv),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        const int k = cq + 4 * j;
+        if (k < NR && k < nrhs && gr < n) {
+          const float xv =
+              cv[j] - (((Red[0][row][k] + Red[1][row][k]) + Red[2][row][k]) + Red[3][row][k]);
+          b[gr + k * ldb] = xv;  // the caller's copy
+          __hip_atomic_store(&tags[gr + (int64_t)k * n],
+                             (1ull << 32) | (unsigned long long)__float_as_uint(xv),
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
       }
       return;
     }
