@@ -753,10 +753,16 @@ __device__ void helper_role(const BlkArgs& g, DiagLds& S, int hx, int nhelp) {
         __hip_atomic_store(g.sync + 4 + 16 * role + b, g.epoch * 16 + s + 1, __ATOMIC_RELEASE,
                            __HIP_MEMORY_SCOPE_AGENT);
     }
-    // far units two at a time (q and q + hw: hw is even, so the same role)
+    // far units three or two at a time (q, q + hw, ...: hw is even, so the
+    // same role)
     for (; q < tot; q += 2 * hw) {
       const int f = q - 2 * ndu, role = f & 1;
-      if (q + hw < tot) {
+      if (q + 2 * hw < tot) {
+        const int64_t v0[3] = {g.nb + (int64_t)NV * (f >> 1), g.nb + (int64_t)NV * ((f + hw) >> 1),
+                               g.nb + (int64_t)NV * ((f + 2 * hw) >> 1)};
+        do_units<MODE, 4, 3>(g, S.tab.role[role], S.pan[role], role, v0, so);
+        q += hw;
+      } else if (q + hw < tot) {
         const int64_t v0[2] = {g.nb + (int64_t)NV * (f >> 1), g.nb + (int64_t)NV * ((f + hw) >> 1)};
         do_units<MODE, 4, 2>(g, S.tab.role[role], S.pan[role], role, v0, so);
       } else {
